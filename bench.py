@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Benchmark of the raster-scan generalised geodesic distance transform on one
+MI355X.
+
+Workload (the reference's headline 3D size): ``generalised_geodesic3d`` on a
+1x1x512x512x512 float32 blob field (12 Gaussian blobs, sigma 8-40 voxels,
++U[0,0.05] noise, default_rng(512)), softmask of ones with 16 seeded zero
+voxels, spacing [1,1,1], v=1e10, lamb=1.0, iter=4.  Inputs are resident in
+HBM before the timed region.  One step = one full call (6 passes x 4
+iterations = 3.22e9 voxel updates).
+
+Prints ONE JSON line on stdout; progress goes to stderr.
+  value      voxel-updates/s of the whole call (N * 6 * iter / wall time)
+  roofline   the dominant kernel (persistent tiled sweep): algorithmic bytes
+             (12 B per voxel update at C=1) per launch / its mean duration,
+             measured with HIP events around each launch on its own stream
+             during the timed region, against MI355X's 8 TB/s HBM peak;
+             traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE +
+             WRITE_SIZE (profiles/traffic_sweep_512.md), or null
+  cpu_baseline  the reference's own OpenMP CPU path (compiled from its
+             sources into oracle/_ref) or, without it, the C restatement
+             (oracle/), on a bounded sample of the same workload
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def read_traffic(path: str):
+    """HBM bytes per sweep launch from a committed rocprofv3 summary."""
+    try:
+        with open(path) as f:
+            text = f.read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(img_gpu, mask_gpu, iters: int):
+    """Times the reference CPU path (or the oracle port) on a bounded sample."""
+    import torch
+
+    sys.path.insert(0, ROOT)
+    import oracle
+
+    sample_d = int(os.environ.get("FG_CPU_SAMPLE_DEPTH", "256"))
+    img = img_gpu[:, :, :sample_d].contiguous().cpu()
+    mask = mask_gpu[:, :, :sample_d].contiguous().cpu()
+    n = img[0, 0].numel()
+    threads = int(os.environ.get("OMP_NUM_THREADS", str(os.cpu_count() or 1)))
+    ref = oracle.reference_module()
+    sample = f"{sample_d}x512x512 slab of the same volume, lamb=1.0, iter={iters}, one run"
+    if ref is not None:
+        kind = "reference"
+        t0 = time.perf_counter()
+        ref.generalised_geodesic3d(img, mask, [1.0, 1.0, 1.0], 1e10, 1.0, 0.0, iters)
+        dt = time.perf_counter() - t0
+    else:
+        kind = "port"
+        t0 = time.perf_counter()
+        oracle.py_geodesic3d(img.numpy(), mask.numpy(), [1.0, 1.0, 1.0], 1e10, 1.0, iters)
+        dt = time.perf_counter() - t0
+        threads = oracle.num_threads()
+    rate = n * 6 * iters / dt
+    return {"value": rate, "unit": "voxel-updates/s", "cores": threads, "kind": kind,
+            "sample": sample + f" ({dt:.2f} s on {cpu_model()})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--iters", type=int, default=4)
+    ap.add_argument("--halo", type=int, default=None, help="override the halo depth K")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build runs the transform on a single MI355X "
+              "(no multi-GPU path); rerun with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    sys.path.insert(0, ROOT)
+    import fastgeodis_amd as FG
+    from fastgeodis_amd import synthetic
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a ROCm GPU", file=sys.stderr)
+        sys.exit(2)
+    if args.halo is not None:
+        t = FG.get_tuning()
+        FG.set_tuning(t["engine"], args.halo, t["max_groups"])
+    tuning = FG.get_tuning()
+
+    S = args.size
+    log(f"[bench] generating {S}^3 input")
+    img, mask = synthetic.headline_512("cuda", S)
+    torch.cuda.synchronize()
+    spacing = [1.0, 1.0, 1.0]
+    N = S ** 3
+    updates = N * 6 * args.iters
+
+    def step():
+        return FG.generalised_geodesic3d(img, mask, spacing, 1e10, 1.0, args.iters)
+
+    for w in range(args.warmup):
+        step()
+        torch.cuda.synchronize()
+        log(f"[bench] warmup {w + 1}/{args.warmup} done")
+
+    FG.FastGeodisCpp.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        out = step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    FG.FastGeodisCpp.profile_enable(False)
+    launches, kern_ms, kern_bytes = FG.FastGeodisCpp.profile_collect()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = updates / (ms_per_step * 1e-3)
+    log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step, {value / 1e9:.1f} Gvu/s")
+
+    avg_ms = kern_ms / max(launches, 1)
+    bytes_per_launch = kern_bytes / max(launches, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if launches else 0.0
+    traffic = read_traffic(os.path.join(ROOT, "profiles", "traffic_sweep_512.md"))
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "sweep_tiled_kernel<4,8,1,lamb1,fast>", "launches": launches,
+                "mean_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": bytes_per_launch}
+    whole_gbs = updates * 12 / (ms_per_step * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("[bench] timing the CPU baseline sample")
+        cpu = cpu_baseline(img, mask, 2)
+        log(f"[bench] cpu baseline {cpu['value'] / 1e9:.3f} Gvu/s ({cpu['kind']})")
+
+    line = {
+        "metric": "voxel-updates/s + % HBM peak, 512^3 geodesic3d; p50/p99 click latency",
+        "value": value,
+        "unit": "voxel-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        # reference's own 512^3 GPU figure: 0.92998 s for 512^3 x 6 x 2 updates
+        # on a Titan X (figures/experiment_3d.json:18) = 1.73e9 voxel-updates/s
+        "vs_baseline": value / (512 ** 3 * 12 / 0.92998),
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"generalised_geodesic3d {S}^3 C=1 lamb=1.0 iter={args.iters} "
+                               "spacing=[1,1,1] v=1e10, 12-blob field, 16 seeds",
+                   "shape": [1, 1, S, S, S], "iter": args.iters, "engine": tuning["engine"],
+                   "halo": tuning["halo"]},
+        "whole_call_alg_GBps": round(whole_gbs, 1),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,202 @@
+"""fastgeodis_amd -- raster-scan generalised geodesic distance transforms on
+AMD Instinct MI355X (gfx950).
+
+Drop-in for the raster-scan entry points of FastGeodis 1.0.5
+(``FastGeodis/__init__.py:37-162, 526-585``): same names, argument names,
+order, defaults and exception types::
+
+    import fastgeodis_amd as FastGeodis
+    d = FastGeodis.generalised_geodesic3d(image, softmask, spacing, v, lamb, iter)
+
+Computation happens in hand-written HIP kernels (``csrc/fg_kernels.hip``)
+behind a C ABI (``include/fastgeodis_hip.h``), reached through the torch
+extension ``FastGeodisCpp`` (``csrc/fg_glue.cpp``).  There is no CPU
+implementation: HIP tensors are processed on their device and the current
+torch stream; CPU tensors are staged through the current HIP device.  If the
+native modules are not built, importing this package raises ImportError.
+
+Additions beyond the reference: :func:`geodesic_batch` (many uneven 2D/3D
+instances in the same persistent launches), :func:`set_tuning`, and the
+interactive-segmentation serving loop in :mod:`fastgeodis_amd.serving`.
+"""
+from __future__ import annotations
+
+import importlib.util
+import os
+import sys
+from typing import List, Optional, Sequence
+
+import torch
+
+__version__ = "0.1.0"
+
+_LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
+
+
+def _load_native():
+    import sysconfig
+
+    path = os.path.join(_LIB_DIR, "FastGeodisCpp" + sysconfig.get_config_var("EXT_SUFFIX"))
+    if not os.path.isfile(path):
+        raise ImportError(
+            f"fastgeodis_amd native module not built ({path} missing); run "
+            "`python -m fastgeodis_amd._build` (hipcc for gfx950)")
+    spec = importlib.util.spec_from_file_location("FastGeodisCpp", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    sys.modules.setdefault("FastGeodisCpp", mod)
+    return mod
+
+
+FastGeodisCpp = _load_native()
+
+
+def generalised_geodesic2d(
+    image: torch.Tensor, softmask: torch.Tensor, v: float, lamb: float, iter: int = 2
+):
+    r"""Generalised geodesic distance, 2D (reference ``FastGeodis/__init__.py:37-65``).
+
+    Args:
+        image: (1, C, H, W) float32 image, any number of channels.
+        softmask: (1, Cm, H, W) float32 soft mask in [0, 1]; 0 marks seeds.
+        v: weight between the unary (mask) and spatial distances.
+        lamb: 0.0 gives Euclidean distance, 1.0 geodesic distance.
+        iter: number of iterations of the raster passes.
+
+    Returns:
+        (1, Cm, H, W) float32 distances (channel 0 transformed).
+    """
+    return FastGeodisCpp.generalised_geodesic2d(image, softmask, v, lamb, 1 - lamb, iter)
+
+
+def generalised_geodesic3d(
+    image: torch.Tensor,
+    softmask: torch.Tensor,
+    spacing: List,
+    v: float,
+    lamb: float,
+    iter: int = 4,
+):
+    r"""Generalised geodesic distance, 3D (reference ``FastGeodis/__init__.py:68-98``).
+
+    ``spacing`` is (s_D, s_H, s_W).  Other arguments as in
+    :func:`generalised_geodesic2d`; tensors are (1, C, D, H, W).
+    """
+    return FastGeodisCpp.generalised_geodesic3d(image, softmask, spacing, v, lamb, 1 - lamb, iter)
+
+
+def signed_generalised_geodesic2d(
+    image: torch.Tensor, softmask: torch.Tensor, v: float, lamb: float, iter: int = 2
+):
+    r"""Signed generalised geodesic distance D(M) - D(1 - M), 2D
+    (reference ``FastGeodis/__init__.py:101-129``)."""
+    return FastGeodisCpp.signed_generalised_geodesic2d(image, softmask, v, lamb, 1 - lamb, iter)
+
+
+def signed_generalised_geodesic3d(
+    image: torch.Tensor,
+    softmask: torch.Tensor,
+    spacing: List,
+    v: float,
+    lamb: float,
+    iter: int = 4,
+):
+    r"""Signed generalised geodesic distance, 3D (reference ``FastGeodis/__init__.py:132-162``)."""
+    return FastGeodisCpp.signed_generalised_geodesic3d(image, softmask, spacing, v, lamb, 1 - lamb, iter)
+
+
+def GSF2d(
+    image: torch.Tensor,
+    softmask: torch.Tensor,
+    theta: float,
+    v: float,
+    lamb: float,
+    iter: int,
+):
+    r"""Geodesic symmetric filtering, 2D (reference ``FastGeodis/__init__.py:526-553``)."""
+    return FastGeodisCpp.GSF2d(image, softmask, theta, v, lamb, iter)
+
+
+def GSF3d(
+    image: torch.Tensor,
+    softmask: torch.Tensor,
+    theta: float,
+    spacing: List,
+    v: float,
+    lamb: float,
+    iter: int,
+):
+    r"""Geodesic symmetric filtering, 3D (reference ``FastGeodis/__init__.py:556-585``)."""
+    return FastGeodisCpp.GSF3d(image, softmask, theta, spacing, v, lamb, iter)
+
+
+# ---- additions -------------------------------------------------------------
+
+OP_UNSIGNED, OP_SIGNED, OP_GSF = 0, 1, 2
+
+
+class Instance:
+    """One member of a batched call.
+
+    op: "unsigned" | "signed" | "gsf".  For "unsigned"/"signed" the weights
+    follow the Python wrappers (l_grad = lamb, l_eucl = 1 - lamb in double);
+    for "gsf" lamb is passed raw and 1 - lamb is formed in float, as the
+    reference's C++ GSF does.
+    """
+
+    __slots__ = ("image", "softmask", "spacing", "v", "lamb", "iter", "op", "theta")
+
+    def __init__(self, image, softmask, v, lamb, iter=None, spacing=None, op="unsigned", theta=0.0):
+        self.image, self.softmask, self.v, self.lamb = image, softmask, v, lamb
+        nd = image.dim() - 2
+        self.iter = iter if iter is not None else (2 if nd == 2 else 4)
+        self.spacing = list(spacing) if spacing is not None else [1.0, 1.0, 1.0]
+        if op not in ("unsigned", "signed", "gsf"):
+            raise ValueError(f"unknown op {op!r}")
+        self.op, self.theta = op, theta
+
+
+def geodesic_batch(instances: Sequence[Instance]) -> List[torch.Tensor]:
+    """Runs many 2D/3D instances of uneven sizes in the same persistent launches.
+
+    All tensors must be on one HIP device.  Each returned tensor equals the
+    single-instance call with the same arguments, bit for bit.
+    """
+    ops = {"unsigned": OP_UNSIGNED, "signed": OP_SIGNED, "gsf": OP_GSF}
+    return FastGeodisCpp.geodesic_batch(
+        [i.image for i in instances],
+        [i.softmask for i in instances],
+        [[float(s) for s in i.spacing] if i.image.dim() == 5 else [] for i in instances],
+        [ops[i.op] for i in instances],
+        [float(i.v) for i in instances],
+        [float(i.lamb) for i in instances],
+        [(1 - float(i.lamb)) if i.op != "gsf" else 0.0 for i in instances],
+        [float(i.theta) for i in instances],
+        [int(i.iter) for i in instances],
+    )
+
+
+def set_tuning(engine: int = 1, halo: int = 4, max_groups: int = 0) -> None:
+    """Engine (0 per-slice, 1 persistent tiled), halo depth K and workgroup
+    cap used by subsequent calls on this host thread."""
+    FastGeodisCpp.set_tuning(int(engine), int(halo), int(max_groups))
+
+
+def get_tuning():
+    e, h, g = FastGeodisCpp.get_tuning()
+    return {"engine": e, "halo": h, "max_groups": g}
+
+
+def native_library_paths() -> List[str]:
+    """Paths of the native libraries this package loads."""
+    import sysconfig
+
+    return [os.path.join(_LIB_DIR, "libfastgeodis_hip.so"),
+            os.path.join(_LIB_DIR, "FastGeodisCpp" + sysconfig.get_config_var("EXT_SUFFIX"))]
+
+
+__all__ = [
+    "generalised_geodesic2d", "generalised_geodesic3d",
+    "signed_generalised_geodesic2d", "signed_generalised_geodesic3d",
+    "GSF2d", "GSF3d", "Instance", "geodesic_batch", "set_tuning", "get_tuning",
+]

@@ -1,0 +1,1499 @@
+// fg_kernels.hip -- gfx950 kernels and C ABI of the raster-scan generalised
+// geodesic distance transform (see include/fastgeodis_hip.h).
+//
+// Semantics (reference FastGeodis v1.0.5):
+//   candidate  cur = fl(fl(d_q + fl(l_eucl*LD)) + fl(l_grad*l)),  l = sum_c |I_c(p) - I_c(q)|
+//              (fastgeodis_cpu.cpp:88-110, 257-277)
+//   update     d(p) = min(d(p), cur) keeping d(p) unless cur is strictly smaller
+//   passes     2D: rows then columns (fastgeodis_cpu.cpp:177-198)
+//              3D: D, H, W pass pairs with per-pass step tables (fastgeodis_cpu.cpp:220-231, 359-404)
+//
+// Layout in HBM.  Every sweep runs along a non-contiguous axis A with lanes on
+// the contiguous axis E and wave rows on the remaining axis B:
+//   D pass: A=D, B=H, E=W on the native (D,H,W) array
+//   H pass: A=H, B=D, E=W on the native array
+//   W pass: A=W, B=H, E=D on a transposed (W,H,D) copy (image transposed once
+//           per call, distance transposed in and out around the W pass pair)
+// B and E are the reference's in-plane axes in its neighbour order (B outer,
+// E inner), so the exact-order path visits candidates in the reference order.
+//
+// Engines.
+//   tiled (default): one persistent, cooperatively launched kernel per sweep.
+//     A workgroup owns a tile of the slice plane and walks along A.  Its
+//     front (the previous slice) lives in VGPRs: lane = E position, register
+//     = B row; E neighbours arrive through DPP wave shifts folded into the
+//     arithmetic, B neighbours are other registers or, across waves, LDS.
+//     The window is the tile grown by K on every tiled side; every K slices a
+//     tile publishes its own values as (value, tag) pairs with device-scope
+//     relaxed 64-bit stores and reloads its K-wide halo from its neighbours'
+//     publications, spinning on the tag (bounded; expiry sets a status word).
+//     Between exchanges the halo is recomputed redundantly: values at window
+//     positions more than j slices from the window edge are exact at step j,
+//     so the tile's own positions are exact for all K steps.
+//   per-slice: one small launch per slice (any channel count); kept as the
+//     fallback and as an independent second path for tests.
+//
+// Arithmetic.  Compiled with -ffp-contract=off (no FMA) and without fast-math.
+// With l_grad and every l_eucl*LD non-negative (sign bit clear), no candidate
+// can be -0, so min over candidates with IEEE minNum (v_min3_f32, NaN
+// candidates ignored) followed by "(m < d) ? m : d" equals the reference's
+// sequential std::min fold bit for bit; otherwise the exact path folds the
+// candidates one by one in the reference order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fastgeodis_hip.h"
+
+namespace {
+
+constexpr float kInf = __builtin_huge_valf();
+constexpr int kMaxK = 16;
+
+// ---------------------------------------------------------------------------
+// Descriptors
+// ---------------------------------------------------------------------------
+
+struct SweepDesc {
+  const float *img;  // image channel 0 in this sweep's layout
+  float *d;          // distance in this sweep's layout
+  long long cs;      // image channel stride (elements)
+  long long sA, sB;  // slice stride and row stride; lanes are contiguous
+  int nA, nB, nE;
+  int dir;  // +1 forward, -1 backward
+  int C;
+  float lg;
+  float e1[9];  // fl(l_eucl * LD[i*3+j]), i = row offset + 1, j = lane offset + 1
+  int nTB, nTE;  // tile grid
+  int TB, TE;    // own tile extent
+  int tile0;     // first tile of this descriptor within its launch
+  long long halo0;  // first halo entry of this descriptor
+};
+
+struct XposeDesc {  // out[c][w][h][d] = in[c][d][h][w]
+  const float *src;
+  float *dst;
+  int C, D, H, W;
+  int tiles_dw;  // tiles per (c, h) plane
+  int tile0;
+};
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+
+// Lane l receives lane l-1 (lane 0 receives 0).
+__device__ __forceinline__ float from_left(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xF, 0xF, false));
+}
+// Lane l receives lane l+1 (lane 63 receives 0).
+__device__ __forceinline__ float from_right(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, false));
+}
+template <int J>
+__device__ __forceinline__ float shiftE(float x) {
+  if constexpr (J == 0) return from_left(x);
+  else if constexpr (J == 2) return from_right(x);
+  else return x;
+}
+
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of tiles so that most neighbours share an L2.
+__device__ __forceinline__ int xcd_remap(int g, int G) {
+  constexpr int nx = 8;
+  const int x = g % nx, i = g / nx;
+  const int q = G / nx, rem = G % nx;
+  return x * q + min(x, rem) + i;
+}
+
+__device__ __forceinline__ unsigned long long pack(float v, unsigned tag) {
+  return ((unsigned long long)tag << 32) | (unsigned long long)__float_as_uint(v);
+}
+
+// One candidate for the element at this lane from the front neighbour
+// (already shifted to this lane).
+template <int CT, bool LAMB1>
+__device__ __forceinline__ float candidate(float dq, const float (&Iq)[CT], const float (&In)[CT],
+                                           float e1, float lg) {
+  float l = fabsf(In[0] - Iq[0]);
+#pragma unroll
+  for (int c = 1; c < CT; ++c) l = l + fabsf(In[c] - Iq[c]);
+  if constexpr (LAMB1) {
+    // l_grad == 1 and l_eucl*LD == +0: fl(1*l) == l and dropping "+ 0" is exact here
+    return dq + l;
+  } else {
+    return (dq + e1) + lg * l;
+  }
+}
+
+// New value of one row element.  dq[i] / Iq[i][c]: front rows at B offset
+// i-1 (unshifted); In[c] / dinit: the element's own new-slice values.
+template <int CT, bool LAMB1, bool FAST>
+__device__ __forceinline__ float relax(const float (&dq)[3], const float (&Iq)[3][CT],
+                                       const float (&In)[CT], float dinit, const float (&e1)[9],
+                                       float lg) {
+  float c[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    float s0[CT], s1[CT], s2[CT];
+#pragma unroll
+    for (int ch = 0; ch < CT; ++ch) {
+      s0[ch] = shiftE<0>(Iq[i][ch]);
+      s1[ch] = Iq[i][ch];
+      s2[ch] = shiftE<2>(Iq[i][ch]);
+    }
+    c[i * 3 + 0] = candidate<CT, LAMB1>(shiftE<0>(dq[i]), s0, In, e1[i * 3 + 0], lg);
+    c[i * 3 + 1] = candidate<CT, LAMB1>(dq[i], s1, In, e1[i * 3 + 1], lg);
+    c[i * 3 + 2] = candidate<CT, LAMB1>(shiftE<2>(dq[i]), s2, In, e1[i * 3 + 2], lg);
+  }
+  if constexpr (FAST) {
+    float m = fminf(fminf(c[0], c[1]), c[2]);
+    m = fminf(fminf(m, c[3]), c[4]);
+    m = fminf(fminf(m, c[5]), c[6]);
+    m = fminf(fminf(m, c[7]), c[8]);
+    return (m < dinit) ? m : dinit;
+  } else {
+    float nd = dinit;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) nd = (c[k] < nd) ? c[k] : nd;
+    return nd;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tiled persistent sweep kernel
+// ---------------------------------------------------------------------------
+
+template <int NW, int R, int CT>
+struct Front {
+  float d[R];           // front distance, own wave rows
+  float I[CT][R + 2];   // front image, rows -1..R relative to the wave
+};
+
+template <int R, int CT>
+struct Slab {            // one prefetched slice
+  float I[CT][R + 2];
+  float d[R];
+};
+
+template <int NW, int R, int CT, bool LAMB1, bool FAST>
+__global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
+    const SweepDesc *__restrict__ descs, int ndesc, int K, unsigned tag_base,
+    unsigned long long *__restrict__ halo, int *__restrict__ status, int spin_limit) {
+  constexpr int WR = NW * R;
+  __shared__ float edge_first[2][NW][64];
+  __shared__ float edge_last[2][NW][64];
+
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int tile = xcd_remap(blockIdx.x, gridDim.x);
+
+  // descriptor owning this tile (binary search on tile0)
+  int lo = 0, hi = ndesc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (descs[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
+  }
+  const SweepDesc &Dd = descs[lo];
+  const float *__restrict__ img = Dd.img;
+  float *__restrict__ dist = Dd.d;
+  const long long cs = Dd.cs, sA = Dd.sA, sB = Dd.sB;
+  const int nA = Dd.nA, nB = Dd.nB, nE = Dd.nE, dir = Dd.dir;
+  const float lg = Dd.lg;
+  float e1[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) e1[k] = Dd.e1[k];
+  const int nTB = Dd.nTB, nTE = Dd.nTE, TB = Dd.TB, TE = Dd.TE;
+  const int lt = tile - Dd.tile0;
+  const int tb = lt / nTE, te = lt - tb * nTE;
+  const int b_own0 = tb * TB, b_own1 = min(nB, b_own0 + TB);
+  const int e_own0 = te * TE, e_own1 = min(nE, e_own0 + TE);
+  const int bw0 = (nTB > 1) ? b_own0 - K : 0;
+  const int ew0 = (nTE > 1) ? e_own0 - K : -1;
+  const int e = ew0 + lane;
+  const bool lane_valid = (e >= 0) && (e < nE);
+  const bool lane_own = (e >= e_own0) && (e < e_own1);
+  const bool pad_lanes = (ew0 < 0) || (ew0 + 63 >= nE);
+  const int own_w = e_own1 - e_own0;
+  const unsigned long long *halo_ro = halo;
+  unsigned long long *my_halo = halo + Dd.halo0 + (long long)lt * 2 * TB * TE;
+
+  const int nsteps = nA - 1;
+  if (nsteps <= 0) return;
+  const int a_first = (dir > 0) ? 1 : nA - 2;
+
+  // row bookkeeping (uniform per wave)
+  int rb[R + 2];  // global B index of rows -1..R relative to the wave
+#pragma unroll
+  for (int r = 0; r < R + 2; ++r) rb[r] = bw0 + w * R + r - 1;
+
+  auto row_ok = [&](int b) { return b >= 0 && b < nB; };
+
+  auto load_slab = [&](Slab<R, CT> &S, int a) {
+    const long long base = (long long)a * sA;
+#pragma unroll
+    for (int r = 0; r < R + 2; ++r) {
+      const int b = rb[r];
+      const bool ok = row_ok(b) && lane_valid;
+      const long long off = base + (long long)b * sB + e;
+#pragma unroll
+      for (int c = 0; c < CT; ++c) S.I[c][r] = ok ? img[c * cs + off] : 0.0f;
+      if (r >= 1 && r <= R) S.d[r - 1] = ok ? dist[off] : kInf;
+    }
+  };
+
+  Front<NW, R, CT> F;
+  {
+    const int af = a_first - dir;  // front slice: not modified by this sweep
+    Slab<R, CT> S;
+    load_slab(S, af);
+#pragma unroll
+    for (int r = 0; r < R; ++r) F.d[r] = S.d[r];
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+#pragma unroll
+      for (int r = 0; r < R + 2; ++r) F.I[c][r] = S.I[c][r];
+  }
+
+  Slab<R, CT> S0, S1;
+  load_slab(S0, a_first);
+  if (nsteps > 1) load_slab(S1, a_first + dir);
+
+  auto step = [&](Slab<R, CT> &S, int s, int slot) {
+    const int a = a_first + s * dir;
+    // wave-boundary front rows through LDS
+    float top = kInf, bot = kInf;
+    if constexpr (NW > 1) {
+      edge_first[slot][w][lane] = F.d[0];
+      edge_last[slot][w][lane] = F.d[R - 1];
+      __syncthreads();
+      if (w > 0) top = edge_last[slot][w - 1][lane];
+      if (w < NW - 1) bot = edge_first[slot][w + 1][lane];
+    }
+    float nd[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      float dq[3];
+      dq[0] = (r == 0) ? top : F.d[r - 1];
+      dq[1] = F.d[r];
+      dq[2] = (r == R - 1) ? bot : F.d[r + 1];
+      float Iq[3][CT], In[CT];
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        Iq[0][c] = F.I[c][r];
+        Iq[1][c] = F.I[c][r + 1];
+        Iq[2][c] = F.I[c][r + 2];
+        In[c] = S.I[c][r + 1];
+      }
+      float v = relax<CT, LAMB1, FAST>(dq, Iq, In, S.d[r], e1, lg);
+      if (pad_lanes) v = lane_valid ? v : kInf;
+      nd[r] = row_ok(rb[r + 1]) ? v : kInf;
+    }
+    // own outputs
+    const long long base = (long long)a * sA;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int b = rb[r + 1];
+      if (b >= b_own0 && b < b_own1 && lane_own) dist[base + (long long)b * sB + e] = nd[r];
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) F.d[r] = nd[r];
+#pragma unroll
+    for (int c = 0; c < CT; ++c)
+#pragma unroll
+      for (int r = 0; r < R + 2; ++r) F.I[c][r] = S.I[c][r];
+    // prefetch slice s + 2 into the slab just consumed
+    if (s + 2 < nsteps) load_slab(S, a + 2 * dir);
+    // halo exchange every K slices
+    if (((s + 1) % K) == 0 && (s + 1) < nsteps) {
+      const int blk = (s + 1) / K;
+      const unsigned tag = tag_base + (unsigned)blk;
+      const int hslot = blk & 1;
+      unsigned long long *pub = my_halo + (long long)hslot * TB * TE;
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int b = rb[r + 1];
+        if (b >= b_own0 && b < b_own1 && lane_own)
+          __hip_atomic_store(pub + (long long)(b - b_own0) * own_w + (e - e_own0), pack(F.d[r], tag),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const int b = rb[r + 1];
+        const bool inside = row_ok(b) && lane_valid;
+        const bool own = b >= b_own0 && b < b_own1 && lane_own;
+        if (inside && !own) {
+          const int otb = b / TB, ote = e / TE;
+          const int ob0 = otb * TB, oe0 = ote * TE;
+          const int ow = min(nE, oe0 + TE) - oe0;
+          const unsigned long long *src = halo_ro + Dd.halo0 +
+                                          (long long)(otb * nTE + ote) * 2 * TB * TE +
+                                          (long long)hslot * TB * TE + (long long)(b - ob0) * ow +
+                                          (e - oe0);
+          unsigned long long x;
+          int spins = 0;
+          while (true) {
+            x = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if ((unsigned)(x >> 32) == tag) break;
+            if (++spins > spin_limit) {
+              __hip_atomic_store(status, FG_EWATCHDOG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              break;
+            }
+          }
+          F.d[r] = __uint_as_float((unsigned)x);
+        }
+      }
+    }
+  };
+
+  int s = 0;
+  for (; s + 1 < nsteps; s += 2) {
+    step(S0, s, 0);
+    step(S1, s + 1, 1);
+  }
+  if (s < nsteps) step(S0, s, 0);
+}
+
+// ---------------------------------------------------------------------------
+// Per-slice engine: one launch updates one slice from the previous one.
+// Visits candidates in the reference order with the reference's min rule.
+// ---------------------------------------------------------------------------
+
+struct SliceDesc {
+  const float *img;
+  float *d;
+  long long cs, sA, sB;
+  int nB, nE, C;
+  float lg;
+  float e1[9];
+};
+
+__global__ void slice_kernel(SliceDesc D, int a, int aq) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long nplane = (long long)D.nB * D.nE;
+  if (idx >= nplane) return;
+  const int b = (int)(idx / D.nE), e = (int)(idx - (long long)b * D.nE);
+  const long long p = (long long)a * D.sA + (long long)b * D.sB + e;
+  float nd = D.d[p];
+  for (int i = 0; i < 3; ++i) {
+    const int bq = b + i - 1;
+    if (bq < 0 || bq >= D.nB) continue;
+    for (int j = 0; j < 3; ++j) {
+      const int eq = e + j - 1;
+      if (eq < 0 || eq >= D.nE) continue;
+      const long long q = (long long)aq * D.sA + (long long)bq * D.sB + eq;
+      float l = 0.0f;
+      if (D.C == 1) {
+        l = fabsf(D.img[p] - D.img[q]);
+      } else {
+        for (int c = 0; c < D.C; ++c) l += fabsf(D.img[c * D.cs + p] - D.img[c * D.cs + q]);
+      }
+      const float cur = (D.d[q] + D.e1[i * 3 + j]) + D.lg * l;
+      nd = (cur < nd) ? cur : nd;
+    }
+  }
+  D.d[p] = nd;
+}
+
+// ---------------------------------------------------------------------------
+// Transposes (batched): out[c][w][h][d] = in[c][d][h][w]
+// ---------------------------------------------------------------------------
+
+constexpr int kXT = 64;
+
+__global__ __launch_bounds__(256) void xpose_kernel(const XposeDesc *__restrict__ descs, int ndesc) {
+  __shared__ float tile[kXT][kXT + 1];
+  const int t = blockIdx.x;
+  int lo = 0, hi = ndesc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (descs[mid].tile0 <= t) lo = mid; else hi = mid - 1;
+  }
+  const XposeDesc &X = descs[lo];
+  const int lt = t - X.tile0;
+  const int plane = lt / X.tiles_dw, tt = lt - plane * X.tiles_dw;
+  const int c = plane / X.H, h = plane - c * X.H;
+  const int ntw = (X.W + kXT - 1) / kXT;
+  const int td = tt / ntw, tw = tt - td * ntw;
+  const int d0 = td * kXT, w0 = tw * kXT;
+  const long long HW = (long long)X.H * X.W, HD = (long long)X.H * X.D;
+  const long long cn = (long long)X.D * HW;
+  const float *src = X.src + c * cn + (long long)h * X.W;
+  float *dst = X.dst + c * cn + (long long)h * X.D;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  for (int r = ty; r < kXT; r += 4) {
+    const int d = d0 + r, w = w0 + tx;
+    if (d < X.D && w < X.W) tile[r][tx] = src[(long long)d * HW + w];
+  }
+  __syncthreads();
+  for (int r = ty; r < kXT; r += 4) {
+    const int w = w0 + r, d = d0 + tx;
+    if (d < X.D && w < X.W) dst[(long long)w * HD + d] = tile[tx][r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise init / epilogue kernels (batched through small descriptors)
+// ---------------------------------------------------------------------------
+
+enum EwKind : int {
+  EW_INIT_UNSIGNED = 0,  // out = v*m (all channels)
+  EW_INIT_SIGNED = 1,    // out = v*m (all channels); a = v*(1-m) (channel 0)
+  EW_FINAL_SIGNED = 2,   // out0 -= a;  out_c -= v*(1-m_c) for c >= 1
+  EW_GSF_MID = 3,        // from (out0, a): thresholds -> inits of the 4 stage-2 transforms
+  EW_GSF_FINAL = 4,      // out0 = -(a - b) + (c2 - out0); channels >= 1 in closed form
+  EW_FILL_U64 = 5,       // zero a range of 64-bit words
+};
+
+struct EwDesc {
+  int kind;
+  long long n0;   // voxels per channel
+  int Cm;
+  const float *mask;
+  float *out;
+  float *a, *b, *c2;
+  float v, theta;
+  unsigned long long *u64;
+  long long nu64;
+  long long elem0;  // first element of this descriptor in the launch
+  long long nelem;
+};
+
+__device__ __forceinline__ float signed_closed(float v, float m) {
+  // untransformed channels of signed: v*m - v*(1-m)
+  return v * m - v * (1.0f - m);
+}
+
+__global__ __launch_bounds__(256) void ew_kernel(const EwDesc *__restrict__ descs, int ndesc,
+                                                  long long total) {
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+       g += (long long)gridDim.x * blockDim.x) {
+    int lo = 0, hi = ndesc - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (descs[mid].elem0 <= g) lo = mid; else hi = mid - 1;
+    }
+    const EwDesc &E = descs[lo];
+    const long long i = g - E.elem0;
+    const float v = E.v;
+    switch (E.kind) {
+      case EW_INIT_UNSIGNED:
+        E.out[i] = v * E.mask[i];
+        break;
+      case EW_INIT_SIGNED: {
+        const float m = E.mask[i];
+        E.out[i] = v * m;
+        if (i < E.n0) E.a[i] = v * (1.0f - m);
+        break;
+      }
+      case EW_FINAL_SIGNED:
+        if (i < E.n0) E.out[i] = E.out[i] - E.a[i];
+        else E.out[i] = E.out[i] - v * (1.0f - E.mask[i]);
+        break;
+      case EW_GSF_MID: {
+        // Ds = D(M) - D(1-M); Md = Ds > theta; Me = Ds > -theta
+        const float ds = E.out[i] - E.a[i];
+        const float md = (ds > E.theta) ? 1.0f : 0.0f;
+        const float me = (ds > -E.theta) ? 1.0f : 0.0f;
+        const float m1 = 1.0f - md;        // mask of signed(image, 1 - Md)
+        E.a[i] = v * m1;                   // D(1-Md) init
+        E.b[i] = v * (1.0f - m1);          // D(1-(1-Md)) init
+        E.c2[i] = v * me;                  // D(Me) init
+        E.out[i] = v * (1.0f - me);        // D(1-Me) init
+        break;
+      }
+      case EW_GSF_FINAL:
+        if (i < E.n0) {
+          E.out[i] = -(E.a[i] - E.b[i]) + (E.c2[i] - E.out[i]);
+        } else {
+          const float m = E.mask[i];
+          const float ds = signed_closed(v, m);
+          const float md = (ds > E.theta) ? 1.0f : 0.0f;
+          const float me = (ds > -E.theta) ? 1.0f : 0.0f;
+          const float dd = -signed_closed(v, 1.0f - md);
+          const float de = signed_closed(v, me);
+          E.out[i] = dd + de;
+        }
+        break;
+      case EW_FILL_U64:
+        E.u64[i] = 0ull;
+        break;
+      default:
+        break;
+    }
+  }
+}
+
+__global__ void fill_i32_kernel(int *p, int v) { *p = v; }
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+thread_local std::string g_err;
+thread_local fg_tuning g_tune = {1, 4, 0};
+
+// Per-launch timing of tiled sweeps (fg_profile_enable / fg_profile_collect).
+struct ProfRec {
+  hipEvent_t start, stop;
+  double bytes;
+};
+thread_local bool g_prof_on = false;
+thread_local std::vector<ProfRec> g_prof;
+
+int fail(int code, const std::string &msg) {
+  g_err = msg;
+  return code;
+}
+
+#define FG_HIP(x)                                                                \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess)                                                        \
+      return fail(FG_EHIP, std::string(#x " failed: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// Step table of one pass: sqrt((s0 + |i-1|*s1) + |j-1|*s2) in float, in this
+// summation order (reference fastgeodis_cpu.cpp:220-231).
+void table3d(float s0, float s1, float s2, float ld[9]) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      float v = s0;
+      v += (float)std::abs(i - 1) * s1;
+      v += (float)std::abs(j - 1) * s2;
+      ld[i * 3 + j] = std::sqrt(v);
+    }
+}
+
+void table2d(float ld[9]) {  // {sqrt2, 1, sqrt2} (reference fastgeodis_cpu.cpp:67)
+  const float r2 = std::sqrt(2.0f);
+  const float row[3] = {r2, 1.0f, r2};
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) ld[i * 3 + j] = row[j];
+}
+
+int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// One unsigned transform to run.
+struct Xform {
+  const float *img;   // native (C, D, H, W) or (C, H, W)
+  const float *imgT;  // transposed (C, W, H, D) or (C, W, H)
+  float *d;           // native layout
+  float *dT;          // transposed scratch
+  int C, D, H, W;
+  bool is2d;
+  float lg;
+  float e1[3][9];  // per pass: D, H, W (2D: [1] rows pass, [2] columns pass)
+  bool lamb1, fast;
+  int iters;
+};
+
+// Tiled plan for one sweep plane.
+struct Plan {
+  int NW, R;
+  int nTB, nTE, TB, TE;
+  bool ok;
+};
+
+struct Shape {
+  int NW, R;
+};
+constexpr Shape kShapes[] = {{1, 1}, {4, 8}};
+
+Plan plan_plane(int nB, int nE, int K) {
+  Plan p{};
+  p.ok = false;
+  const Shape s = (nB == 1) ? kShapes[0] : kShapes[1];
+  p.NW = s.NW;
+  p.R = s.R;
+  const int WR = s.NW * s.R;
+  if (nE <= 62) {
+    p.nTE = 1;
+    p.TE = nE;
+  } else {
+    const int cap = 64 - 2 * K;
+    if (cap < 1) return p;
+    p.nTE = ceil_div(nE, cap);
+    p.TE = ceil_div(nE, p.nTE);
+  }
+  if (nB <= WR) {
+    p.nTB = 1;
+    p.TB = nB;
+  } else {
+    const int cap = WR - 2 * K;
+    if (cap < 1) return p;
+    p.nTB = ceil_div(nB, cap);
+    p.TB = ceil_div(nB, p.nTB);
+  }
+  p.ok = true;
+  return p;
+}
+
+// (nB, nE) of each pass's sweep plane: 0 = D, 1 = H (2D: rows), 2 = W (2D: columns)
+void pass_plane(const Xform &x, int pass, int &nA, int &nB, int &nE) {
+  if (x.is2d) {
+    nB = 1;
+    if (pass == 1) { nA = x.H; nE = x.W; }
+    else if (pass == 2) { nA = x.W; nE = x.H; }
+    else { nA = 1; nE = x.W; }
+    return;
+  }
+  if (pass == 0) { nA = x.D; nB = x.H; nE = x.W; }
+  else if (pass == 1) { nA = x.H; nB = x.D; nE = x.W; }
+  else { nA = x.W; nB = x.H; nE = x.D; }
+}
+
+template <int NW, int R, int CT, bool L1, bool FA>
+const void *kernel_ptr() {
+  return reinterpret_cast<const void *>(&sweep_tiled_kernel<NW, R, CT, L1, FA>);
+}
+
+template <int NW, int R>
+const void *pick_kernel_nr(int C, bool lamb1, bool fast) {
+#define FG_K(CT)                                                                               \
+  if (C == CT) {                                                                               \
+    if (fast) return lamb1 ? kernel_ptr<NW, R, CT, true, true>() : kernel_ptr<NW, R, CT, false, true>(); \
+    return lamb1 ? kernel_ptr<NW, R, CT, true, false>() : kernel_ptr<NW, R, CT, false, false>(); \
+  }
+  FG_K(1)
+  FG_K(2)
+  FG_K(3)
+  FG_K(4)
+#undef FG_K
+  return nullptr;
+}
+
+const void *pick_kernel(int NW, int R, int C, bool lamb1, bool fast) {
+  if (NW == 1 && R == 1) return pick_kernel_nr<1, 1>(C, lamb1, fast);
+  if (NW == 4 && R == 8) return pick_kernel_nr<4, 8>(C, lamb1, fast);
+  return nullptr;
+}
+
+// Workspace layout helper: carves 256-byte aligned regions.
+struct Carver {
+  char *base;
+  size_t cap, used = 0;
+  bool dry;
+  Carver(void *b, size_t c, bool dry_run) : base((char *)b), cap(c), dry(dry_run) {}
+  void *take(size_t bytes) {
+    used = (used + 255) & ~(size_t)255;
+    void *p = dry ? nullptr : base + used;
+    used += bytes;
+    return p;
+  }
+  bool fits() const { return used <= cap; }
+};
+
+// Recorded launch.  Descriptor arrays live in one host blob that is copied to
+// the workspace once per call, before any launch is issued.
+struct Cmd {
+  enum Type { TILED, SLICE, XPOSE, EW, STATUS } type;
+  const void *kern = nullptr;
+  int grid = 0, block = 0;
+  size_t off = 0;  // descriptor offset in the blob
+  int ndesc = 0;
+  int K = 0;
+  unsigned tag_base = 0;
+  long long total = 0;  // EW: elements
+  SliceDesc slice{};    // SLICE: descriptor by value
+  int a = 0, aq = 0;
+  double alg_bytes = 0;  // TILED: algorithmic HBM bytes of the launch
+};
+
+struct Recorder {
+  std::vector<unsigned char> blob;
+  std::vector<std::pair<std::string, size_t>> seen;  // identical arrays share storage
+  std::vector<Cmd> cmds;
+
+  size_t put(const void *p, size_t bytes) {
+    std::string key((const char *)p, bytes);
+    for (auto &s : seen)
+      if (s.first == key) return s.second;
+    size_t off = (blob.size() + 255) & ~(size_t)255;
+    blob.resize(off + bytes);
+    std::memcpy(blob.data() + off, p, bytes);
+    seen.emplace_back(std::move(key), off);
+    return off;
+  }
+};
+
+struct Device {
+  int id = -1;
+  int cus = 0;
+};
+
+int query_device(Device &dev) {
+  int id = 0;
+  FG_HIP(hipGetDevice(&id));
+  int cus = 0, coop = 0;
+  FG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, id));
+  FG_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, id));
+  if (!coop) return fail(FG_EHIP, "device does not support cooperative launches");
+  dev.id = id;
+  dev.cus = cus;
+  return FG_OK;
+}
+
+// Occupancy-limited number of co-resident workgroups per kernel (cached per
+// host thread and device).
+int coresident_groups(const void *kern, int threads, const Device &dev, long long &cap) {
+  struct Entry { const void *k; int dev; long long cap; };
+  thread_local std::vector<Entry> cache;
+  for (auto &e : cache)
+    if (e.k == kern && e.dev == dev.id) { cap = e.cap; return FG_OK; }
+  int per_cu = 0;
+  FG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, threads, 0));
+  cap = (long long)per_cu * dev.cus;
+  cache.push_back({kern, dev.id, cap});
+  return FG_OK;
+}
+
+struct SweepJob {
+  Xform *x;
+  int pass;
+};
+
+struct Runner {
+  Device dev;
+  int K;
+  int engine;
+  int max_groups;
+  size_t halo_cap;  // entries
+  unsigned tag_base = 0;
+  Recorder rec;
+
+  void fill_desc(const Xform &x, int pass, int dir, SweepDesc &s) {
+    std::memset(&s, 0, sizeof s);
+    const long long N = (long long)x.D * x.H * x.W;
+    s.C = x.C;
+    s.cs = N;
+    s.dir = dir;
+    s.lg = x.lg;
+    pass_plane(x, pass, s.nA, s.nB, s.nE);
+    if (x.is2d) {
+      if (pass == 2) { s.img = x.imgT; s.d = x.dT; s.sA = x.H; }   // transposed (W, H)
+      else { s.img = x.img; s.d = x.d; s.sA = x.W; }                 // native (H, W)
+      s.sB = 0;
+    } else if (pass == 0) {
+      s.img = x.img; s.d = x.d; s.sA = (long long)x.H * x.W; s.sB = x.W;
+    } else if (pass == 1) {
+      s.img = x.img; s.d = x.d; s.sA = x.W; s.sB = (long long)x.H * x.W;
+    } else {
+      s.img = x.imgT; s.d = x.dT; s.sA = (long long)x.H * x.D; s.sB = x.D;
+    }
+    for (int k = 0; k < 9; ++k) s.e1[k] = x.e1[pass][k];
+  }
+
+  void record_per_slice(const Xform &x, int pass, int dir) {
+    SweepDesc s;
+    fill_desc(x, pass, dir, s);
+    Cmd c;
+    c.type = Cmd::SLICE;
+    SliceDesc &sd = c.slice;
+    sd.img = s.img; sd.d = s.d; sd.cs = s.cs; sd.sA = s.sA; sd.sB = s.sB;
+    sd.nB = s.nB; sd.nE = s.nE; sd.C = s.C; sd.lg = s.lg;
+    for (int k = 0; k < 9; ++k) sd.e1[k] = s.e1[k];
+    c.grid = ceil_div((long long)s.nB * s.nE, 256);
+    c.block = 256;
+    if (dir > 0) {
+      for (int a = 1; a < s.nA; ++a) { c.a = a; c.aq = a - 1; rec.cmds.push_back(c); }
+    } else {
+      for (int a = s.nA - 2; a >= 0; --a) { c.a = a; c.aq = a + 1; rec.cmds.push_back(c); }
+    }
+  }
+
+  int record_tiled(std::vector<SweepJob> &jobs, int dir, int NW, int R, int C, bool lamb1, bool fast) {
+    std::vector<SweepDesc> descs;
+    long long halo_used = 0;
+    int tiles = 0, max_blocks = 0;
+    double alg = 0;
+    for (auto &j : jobs) {
+      SweepDesc s;
+      fill_desc(*j.x, j.pass, dir, s);
+      alg += (double)(s.nA - 1) * s.nB * s.nE * (4.0 * s.C + 8.0);
+      Plan p = plan_plane(s.nB, s.nE, K);
+      s.nTB = p.nTB; s.nTE = p.nTE; s.TB = p.TB; s.TE = p.TE;
+      s.tile0 = tiles;
+      s.halo0 = halo_used;
+      const int nt = p.nTB * p.nTE;
+      tiles += nt;
+      halo_used += (long long)nt * 2 * p.TB * p.TE;
+      max_blocks = std::max(max_blocks, (s.nA - 1) / K + 1);
+      descs.push_back(s);
+    }
+    if (descs.empty()) return FG_OK;
+    if ((size_t)halo_used > halo_cap) return fail(FG_EWORKSPACE, "workspace too small for the sweep plan");
+    if ((unsigned long long)tag_base + max_blocks + 2 >= 0x7fffffffull) {
+      record_zero_halo();
+      tag_base = 0;
+    }
+    Cmd c;
+    c.type = Cmd::TILED;
+    c.kern = pick_kernel(NW, R, C, lamb1, fast);
+    c.grid = tiles;
+    c.block = NW * 64;
+    c.off = rec.put(descs.data(), descs.size() * sizeof(SweepDesc));
+    c.ndesc = (int)descs.size();
+    c.K = K;
+    c.tag_base = tag_base;
+    c.alg_bytes = alg;
+    rec.cmds.push_back(c);
+    tag_base += (unsigned)max_blocks + 1;
+    return FG_OK;
+  }
+
+  void record_zero_halo() {
+    if (halo_cap == 0) return;
+    EwDesc e{};
+    e.kind = EW_FILL_U64;
+    e.u64 = nullptr;  // patched at replay: the halo base
+    e.nelem = (long long)halo_cap;
+    record_ew({e});
+  }
+
+  void record_ew(const std::vector<EwDesc> &in) {
+    std::vector<EwDesc> ds = in;
+    long long total = 0;
+    for (auto &e : ds) {
+      e.elem0 = total;
+      total += e.nelem;
+    }
+    if (total == 0) return;
+    Cmd c;
+    c.type = Cmd::EW;
+    c.off = rec.put(ds.data(), ds.size() * sizeof(EwDesc));
+    c.ndesc = (int)ds.size();
+    c.total = total;
+    c.block = 256;
+    c.grid = (int)std::max<long long>(1, std::min<long long>(ceil_div(total, 256), (long long)dev.cus * 16));
+    rec.cmds.push_back(c);
+  }
+
+  void record_xpose(const std::vector<XposeDesc> &in) {
+    std::vector<XposeDesc> ds = in;
+    int tiles = 0;
+    for (auto &x : ds) {
+      x.tiles_dw = ceil_div(x.D, kXT) * ceil_div(x.W, kXT);
+      x.tile0 = tiles;
+      tiles += x.C * x.H * x.tiles_dw;
+    }
+    if (tiles == 0) return;
+    Cmd c;
+    c.type = Cmd::XPOSE;
+    c.off = rec.put(ds.data(), ds.size() * sizeof(XposeDesc));
+    c.ndesc = (int)ds.size();
+    c.grid = tiles;
+    c.block = 256;
+    rec.cmds.push_back(c);
+  }
+
+  static XposeDesc xd(const float *src, float *dst, int C, int D, int H, int W) {
+    XposeDesc x{};
+    x.src = src; x.dst = dst; x.C = C; x.D = D; x.H = H; x.W = W;
+    return x;
+  }
+  static void native_dims(const Xform &x, int &D, int &H, int &W) {
+    if (x.is2d) { D = x.H; H = 1; W = x.W; }  // (H, W) seen as (D'=H, H'=1, W'=W)
+    else { D = x.D; H = x.H; W = x.W; }
+  }
+
+  // Record one pass (forward then backward) for the active transforms.
+  int sweep_pass(std::vector<Xform *> &active, int pass) {
+    for (int dir : {+1, -1}) {
+      struct G { int NW, R, C; bool l1, fa; std::vector<SweepJob> jobs; };
+      std::vector<G> groups;
+      for (auto *x : active) {
+        int nA, nB, nE;
+        pass_plane(*x, pass, nA, nB, nE);
+        if (nA <= 1) continue;
+        Plan p = plan_plane(nB, nE, K);
+        if (engine == 0 || !p.ok || x->C > 4) {
+          record_per_slice(*x, pass, dir);
+          continue;
+        }
+        bool found = false;
+        for (auto &g : groups)
+          if (g.NW == p.NW && g.R == p.R && g.C == x->C && g.l1 == x->lamb1 && g.fa == x->fast) {
+            g.jobs.push_back({x, pass});
+            found = true;
+            break;
+          }
+        if (!found) groups.push_back({p.NW, p.R, x->C, x->lamb1, x->fast, {{x, pass}}});
+      }
+      for (auto &g : groups) {
+        int rc = record_group(g.jobs, dir, g.NW, g.R, g.C, g.l1, g.fa);
+        if (rc) return rc;
+      }
+    }
+    return FG_OK;
+  }
+
+  // A group is split into launches whose tiles can all be co-resident.
+  int record_group(std::vector<SweepJob> &jobs, int dir, int NW, int R, int C, bool l1, bool fa) {
+    const void *kern = pick_kernel(NW, R, C, l1, fa);
+    if (!kern) return fail(FG_EINVAL, "no tiled kernel for this shape");
+    long long cap = 0;
+    int rc = coresident_groups(kern, NW * 64, dev, cap);
+    if (rc) return rc;
+    if (max_groups > 0) cap = std::min<long long>(cap, max_groups);
+    std::vector<SweepJob> chunk;
+    long long tiles = 0;
+    for (auto &j : jobs) {
+      int nA, nB, nE;
+      pass_plane(*j.x, j.pass, nA, nB, nE);
+      Plan p = plan_plane(nB, nE, K);
+      const long long nt = (long long)p.nTB * p.nTE;
+      if (nt > cap) {  // one plane alone cannot be co-resident: per-slice engine
+        record_per_slice(*j.x, j.pass, dir);
+        continue;
+      }
+      if (tiles + nt > cap) {
+        if ((rc = record_tiled(chunk, dir, NW, R, C, l1, fa))) return rc;
+        chunk.clear();
+        tiles = 0;
+      }
+      chunk.push_back(j);
+      tiles += nt;
+    }
+    if (!chunk.empty()) return record_tiled(chunk, dir, NW, R, C, l1, fa);
+    return FG_OK;
+  }
+
+  int record_run(std::vector<Xform> &xs) {
+    int max_it = 0;
+    for (auto &x : xs) max_it = std::max(max_it, x.iters);
+    for (int it = 0; it < max_it; ++it) {
+      std::vector<Xform *> active;
+      for (auto &x : xs)
+        if (x.iters > it) active.push_back(&x);
+      int rc;
+      if ((rc = sweep_pass(active, 0))) return rc;
+      if ((rc = sweep_pass(active, 1))) return rc;
+      std::vector<XposeDesc> to, back;
+      for (auto *x : active) {
+        int D, H, W;
+        native_dims(*x, D, H, W);
+        to.push_back(xd(x->d, x->dT, 1, D, H, W));
+        back.push_back(xd(x->dT, x->d, 1, W, H, D));
+      }
+      record_xpose(to);
+      if ((rc = sweep_pass(active, 2))) return rc;
+      record_xpose(back);
+    }
+    return FG_OK;
+  }
+
+  int replay(hipStream_t stream, unsigned char *blob_dev, size_t blob_cap, int *status,
+             unsigned long long *halo) {
+    if (rec.blob.size() > blob_cap) return fail(FG_EWORKSPACE, "descriptor area too small");
+    // the halo base is only known now: patch fill descriptors
+    for (auto &c : rec.cmds)
+      if (c.type == Cmd::EW) {
+        EwDesc *ds = reinterpret_cast<EwDesc *>(rec.blob.data() + c.off);
+        for (int i = 0; i < c.ndesc; ++i)
+          if (ds[i].kind == EW_FILL_U64) ds[i].u64 = halo;
+      }
+    if (!rec.blob.empty())
+      FG_HIP(hipMemcpyAsync(blob_dev, rec.blob.data(), rec.blob.size(), hipMemcpyHostToDevice, stream));
+    const int spin = 1 << 22;
+    for (auto &c : rec.cmds) {
+      switch (c.type) {
+        case Cmd::STATUS:
+          hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(1), 0, stream, status, 0);
+          break;
+        case Cmd::TILED: {
+          const SweepDesc *dp = reinterpret_cast<const SweepDesc *>(blob_dev + c.off);
+          int ndesc = c.ndesc, k = c.K, sp = spin;
+          unsigned tb = c.tag_base;
+          void *args[] = {(void *)&dp, &ndesc, &k, &tb, (void *)&halo, (void *)&status, &sp};
+          ProfRec pr{};
+          if (g_prof_on) {
+            FG_HIP(hipEventCreate(&pr.start));
+            FG_HIP(hipEventCreate(&pr.stop));
+            FG_HIP(hipEventRecord(pr.start, stream));
+          }
+          FG_HIP(hipLaunchCooperativeKernel(c.kern, dim3(c.grid), dim3(c.block), args, 0, stream));
+          if (g_prof_on) {
+            FG_HIP(hipEventRecord(pr.stop, stream));
+            pr.bytes = c.alg_bytes;
+            g_prof.push_back(pr);
+          }
+          break;
+        }
+        case Cmd::SLICE:
+          hipLaunchKernelGGL(slice_kernel, dim3(c.grid), dim3(c.block), 0, stream, c.slice, c.a, c.aq);
+          break;
+        case Cmd::XPOSE: {
+          const XposeDesc *dp = reinterpret_cast<const XposeDesc *>(blob_dev + c.off);
+          hipLaunchKernelGGL(xpose_kernel, dim3(c.grid), dim3(c.block), 0, stream, dp, c.ndesc);
+          break;
+        }
+        case Cmd::EW: {
+          const EwDesc *dp = reinterpret_cast<const EwDesc *>(blob_dev + c.off);
+          hipLaunchKernelGGL(ew_kernel, dim3(c.grid), dim3(c.block), 0, stream, dp, c.ndesc, c.total);
+          break;
+        }
+      }
+      FG_HIP(hipGetLastError());
+    }
+    return FG_OK;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Operation planning shared by the single-call and batched entry points
+// ---------------------------------------------------------------------------
+
+struct OpSpec {
+  const float *image;
+  const float *mask;
+  float *out;
+  int C, Cm, D, H, W;
+  bool is2d;
+  int op;
+  int iters;
+  float spacing[3];
+  float v, lg, le, theta;
+};
+
+bool non_negative(float x) { return !std::signbit(x); }
+
+void make_xform(const OpSpec &o, float lg, float le, Xform &x) {
+  x.C = o.C; x.D = o.D; x.H = o.H; x.W = o.W; x.is2d = o.is2d; x.lg = lg; x.iters = o.iters;
+  float ld[3][9];
+  if (o.is2d) {
+    table2d(ld[0]);
+    table2d(ld[1]);
+    table2d(ld[2]);
+  } else {
+    float sq[3];
+    for (int k = 0; k < 3; ++k) sq[k] = o.spacing[k] * o.spacing[k];
+    table3d(sq[0], sq[1], sq[2], ld[0]);  // D pass: in-plane (H, W)
+    table3d(sq[1], sq[0], sq[2], ld[1]);  // H pass: in-plane (D, W)
+    table3d(sq[2], sq[1], sq[0], ld[2]);  // W pass: in-plane (H, D)
+  }
+  bool fast = non_negative(lg);
+  bool e1zero = true;
+  for (int p = 0; p < 3; ++p)
+    for (int k = 0; k < 9; ++k) {
+      x.e1[p][k] = le * ld[p][k];
+      fast = fast && non_negative(x.e1[p][k]);
+      uint32_t bits;
+      std::memcpy(&bits, &x.e1[p][k], 4);
+      e1zero = e1zero && bits == 0u;
+    }
+  x.fast = fast;
+  x.lamb1 = (lg == 1.0f) && e1zero;
+}
+
+long long vox(const OpSpec &o) { return (long long)o.D * o.H * o.W; }
+
+int n_dist(int op) { return op == FG_OP_UNSIGNED ? 1 : (op == FG_OP_SIGNED ? 2 : 4); }
+
+size_t halo_entries_for(const std::vector<OpSpec> &ops, int K) {
+  size_t total = 0;
+  for (auto &o : ops) {
+    Xform x{};
+    x.D = o.D; x.H = o.H; x.W = o.W; x.is2d = o.is2d;
+    size_t worst = 0;
+    for (int p = 0; p < 3; ++p) {
+      int nA, nB, nE;
+      pass_plane(x, p, nA, nB, nE);
+      Plan pl = plan_plane(nB, nE, K);
+      if (!pl.ok) continue;
+      worst = std::max(worst, (size_t)pl.nTB * pl.nTE * 2 * pl.TB * pl.TE);
+    }
+    total += worst * (size_t)n_dist(o.op);
+  }
+  return total;
+}
+
+// Upper bound of the descriptor blob: identical descriptor arrays are stored
+// once, and the active set of transforms changes at most once per distinct
+// iteration count.
+size_t blob_bytes_for(const std::vector<OpSpec> &ops) {
+  size_t ntrans = 0;
+  std::vector<int> its;
+  for (auto &o : ops) {
+    ntrans += n_dist(o.op);
+    if (std::find(its.begin(), its.end(), o.iters) == its.end()) its.push_back(o.iters);
+  }
+  const size_t sets = its.size() + 1;
+  const size_t nsweep = 2 * 6 * sets;  // (pass, dir) x active sets, x2 for GSF stages
+  size_t bytes = 0;
+  bytes += nsweep * (ntrans * sizeof(SweepDesc) + 256) * 2;
+  bytes += (4 * sets + 2) * (ntrans * sizeof(XposeDesc) + 256) + (ops.size() * sizeof(XposeDesc) + 256);
+  bytes += 8 * (ops.size() * sizeof(EwDesc) + 256);
+  return bytes + 4096;
+}
+
+struct Layout {
+  int *status;
+  unsigned char *blob;
+  size_t blob_cap;
+  unsigned long long *halo;
+  size_t halo_cap;
+};
+
+void layout_common(const std::vector<OpSpec> &ops, int K, Layout &L, Carver &cv) {
+  L.status = (int *)cv.take(sizeof(int));
+  L.blob_cap = blob_bytes_for(ops);
+  L.blob = (unsigned char *)cv.take(L.blob_cap);
+  L.halo_cap = halo_entries_for(ops, K);
+  L.halo = (unsigned long long *)cv.take(L.halo_cap * sizeof(unsigned long long) + 8);
+}
+
+size_t op_buffers_bytes(const OpSpec &o) {
+  const long long N = vox(o);
+  const int nd = n_dist(o.op);
+  // image transpose (C*N) + extra distances (nd-1) + transposed distances (nd)
+  const size_t floats = (size_t)o.C * N + (size_t)(nd - 1) * N + (size_t)nd * N;
+  return floats * sizeof(float) + 16 * 256;
+}
+
+size_t workspace_for(const std::vector<OpSpec> &ops, int K) {
+  Layout L{};
+  Carver cv(nullptr, 0, true);
+  layout_common(ops, K, L, cv);
+  for (auto &o : ops) cv.take(op_buffers_bytes(o));
+  return ((cv.used + 255) & ~(size_t)255) + 256;
+}
+
+int validate(const OpSpec &o) {
+  if (!o.image || !o.mask || !o.out) return fail(FG_EINVAL, "null pointer");
+  if (o.C < 1 || o.Cm < 1 || o.D < 1 || o.H < 1 || o.W < 1)
+    return fail(FG_EINVAL, "extents and channel counts must be >= 1");
+  if (o.is2d && o.D != 1) return fail(FG_EINVAL, "2D instance must have D == 1");
+  if (o.op < FG_OP_UNSIGNED || o.op > FG_OP_GSF) return fail(FG_EINVAL, "unknown op");
+  if (vox(o) * std::max(o.C, o.Cm) >= (1ll << 40)) return fail(FG_EINVAL, "volume too large");
+  if (o.H >= (1 << 30) || o.W >= (1 << 30) || o.D >= (1 << 30)) return fail(FG_EINVAL, "extent too large");
+  return FG_OK;
+}
+
+int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream) {
+  for (auto &o : ops) {
+    int rc = validate(o);
+    if (rc) return rc;
+  }
+  const int K = std::max(1, std::min(g_tune.halo, kMaxK));
+  const size_t need = workspace_for(ops, K);
+  if (!ws || wsb < need)
+    return fail(FG_EWORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
+  Layout L{};
+  Carver cv(ws, wsb, false);
+  layout_common(ops, K, L, cv);
+
+  Runner run{};
+  int rc = query_device(run.dev);
+  if (rc) return rc;
+  run.K = K;
+  run.engine = g_tune.engine;
+  run.max_groups = g_tune.max_groups;
+  run.halo_cap = L.halo_cap;
+
+  struct OpBufs { float *imgT; float *dist[4]; float *dT[4]; };
+  std::vector<OpBufs> bufs(ops.size());
+  for (size_t i = 0; i < ops.size(); ++i) {
+    const OpSpec &o = ops[i];
+    const long long N = vox(o);
+    const int nd = n_dist(o.op);
+    const size_t start = cv.used;
+    bufs[i].imgT = (float *)cv.take((size_t)o.C * N * sizeof(float));
+    bufs[i].dist[0] = o.out;
+    for (int k = 1; k < nd; ++k) bufs[i].dist[k] = (float *)cv.take((size_t)N * sizeof(float));
+    for (int k = 0; k < nd; ++k) bufs[i].dT[k] = (float *)cv.take((size_t)N * sizeof(float));
+    cv.used = start;
+    cv.take(op_buffers_bytes(o));
+  }
+  if (!cv.fits()) return fail(FG_EWORKSPACE, "workspace too small");
+
+  // the status word and the halo tags start from zero on every call
+  {
+    Cmd c;
+    c.type = Cmd::STATUS;
+    run.rec.cmds.push_back(c);
+  }
+  run.record_zero_halo();
+
+  // image transposes (once per op)
+  {
+    std::vector<XposeDesc> xs;
+    for (size_t i = 0; i < ops.size(); ++i) {
+      const OpSpec &o = ops[i];
+      if (o.is2d) xs.push_back(Runner::xd(o.image, bufs[i].imgT, o.C, o.H, 1, o.W));
+      else xs.push_back(Runner::xd(o.image, bufs[i].imgT, o.C, o.D, o.H, o.W));
+    }
+    run.record_xpose(xs);
+  }
+
+  // stage 1: initialise and transform
+  std::vector<EwDesc> init;
+  std::vector<Xform> xs;
+  for (size_t i = 0; i < ops.size(); ++i) {
+    const OpSpec &o = ops[i];
+    const long long N = vox(o);
+    EwDesc e{};
+    e.n0 = N;
+    e.Cm = o.Cm;
+    e.mask = o.mask;
+    e.out = o.out;
+    e.v = o.v;
+    e.theta = o.theta;
+    e.nelem = N * o.Cm;
+    float lg = o.lg, le = o.le;
+    if (o.op == FG_OP_GSF) le = 1 - o.lg;  // reference fastgeodis.cpp:321: 1 - lambda in float
+    if (o.op == FG_OP_UNSIGNED) {
+      e.kind = EW_INIT_UNSIGNED;
+    } else {
+      e.kind = EW_INIT_SIGNED;
+      e.a = bufs[i].dist[1];
+    }
+    init.push_back(e);
+    const int nd = (o.op == FG_OP_UNSIGNED) ? 1 : 2;
+    for (int k = 0; k < nd; ++k) {
+      Xform x{};
+      make_xform(o, lg, le, x);
+      x.img = o.image;
+      x.imgT = bufs[i].imgT;
+      x.d = bufs[i].dist[k];
+      x.dT = bufs[i].dT[k];
+      xs.push_back(x);
+    }
+  }
+  run.record_ew(init);
+  if ((rc = run.record_run(xs))) return rc;
+
+  // stage 2 (GSF): thresholds, four more transforms
+  std::vector<EwDesc> mid, fin;
+  std::vector<Xform> xs2;
+  for (size_t i = 0; i < ops.size(); ++i) {
+    const OpSpec &o = ops[i];
+    const long long N = vox(o);
+    EwDesc e{};
+    e.n0 = N;
+    e.Cm = o.Cm;
+    e.mask = o.mask;
+    e.out = o.out;
+    e.v = o.v;
+    e.theta = o.theta;
+    if (o.op == FG_OP_SIGNED) {
+      e.kind = EW_FINAL_SIGNED;
+      e.a = bufs[i].dist[1];
+      e.nelem = N * o.Cm;
+      fin.push_back(e);
+    } else if (o.op == FG_OP_GSF) {
+      EwDesc m = e;
+      m.kind = EW_GSF_MID;
+      m.a = bufs[i].dist[1];
+      m.b = bufs[i].dist[2];
+      m.c2 = bufs[i].dist[3];
+      m.nelem = N;
+      mid.push_back(m);
+      EwDesc f = m;
+      f.kind = EW_GSF_FINAL;
+      f.nelem = N * o.Cm;
+      fin.push_back(f);
+      const float le = 1 - o.lg;
+      // dist[0] = out: D(1-Me); dist[1]: D(1-Md); dist[2]: D(Md); dist[3]: D(Me)
+      for (int k = 0; k < 4; ++k) {
+        Xform x{};
+        make_xform(o, o.lg, le, x);
+        x.img = o.image;
+        x.imgT = bufs[i].imgT;
+        x.d = bufs[i].dist[k];
+        x.dT = bufs[i].dT[k];
+        xs2.push_back(x);
+      }
+    }
+  }
+  if (!mid.empty()) {
+    run.record_ew(mid);
+    if ((rc = run.record_run(xs2))) return rc;
+  }
+  run.record_ew(fin);
+  return run.replay(stream, L.blob, L.blob_cap, L.status, L.halo);
+}
+
+OpSpec spec2d(const float *image, const float *mask, float *out, int C, int Cm, int H, int W, int op,
+              int iters, float v, float lg, float le, float theta) {
+  OpSpec o{};
+  o.image = image; o.mask = mask; o.out = out; o.C = C; o.Cm = Cm; o.D = 1; o.H = H; o.W = W;
+  o.is2d = true; o.op = op; o.iters = iters; o.v = v; o.lg = lg; o.le = le; o.theta = theta;
+  o.spacing[0] = o.spacing[1] = o.spacing[2] = 1.0f;
+  return o;
+}
+
+OpSpec spec3d(const float *image, const float *mask, float *out, int C, int Cm, int D, int H, int W,
+              const float *spacing, int op, int iters, float v, float lg, float le, float theta) {
+  OpSpec o{};
+  o.image = image; o.mask = mask; o.out = out; o.C = C; o.Cm = Cm; o.D = D; o.H = H; o.W = W;
+  o.is2d = false; o.op = op; o.iters = iters; o.v = v; o.lg = lg; o.le = le; o.theta = theta;
+  for (int k = 0; k < 3; ++k) o.spacing[k] = spacing ? spacing[k] : 1.0f;
+  return o;
+}
+
+int run_single(OpSpec o, void *ws, size_t wsb, void *stream) {
+  std::vector<OpSpec> ops{o};
+  return run_ops(ops, ws, wsb, (hipStream_t)stream);
+}
+
+int current_halo() { return std::max(1, std::min(g_tune.halo, kMaxK)); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+int fg_abi_version(void) { return 1; }
+
+const char *fg_last_error(void) { return g_err.c_str(); }
+
+size_t fg_workspace_bytes(int op, int C, int D, int H, int W) {
+  OpSpec o{};
+  o.op = (op < FG_OP_UNSIGNED || op > FG_OP_GSF) ? FG_OP_GSF : op;
+  o.C = std::max(C, 1); o.Cm = 1; o.D = std::max(D, 1); o.H = std::max(H, 1); o.W = std::max(W, 1);
+  o.is2d = (D <= 1);
+  o.iters = 1;
+  std::vector<OpSpec> ops{o};
+  return workspace_for(ops, current_halo());
+}
+
+int fg_geodesic2d_f32(const float *image, const float *mask, float *out, int C, int Cm, int H, int W,
+                      float v, float l_grad, float l_eucl, int iterations, void *workspace,
+                      size_t workspace_bytes, void *stream) {
+  return run_single(spec2d(image, mask, out, C, Cm, H, W, FG_OP_UNSIGNED, iterations, v, l_grad, l_eucl, 0.f),
+                    workspace, workspace_bytes, stream);
+}
+
+int fg_geodesic3d_f32(const float *image, const float *mask, float *out, int C, int Cm, int D, int H, int W,
+                      const float spacing[3], float v, float l_grad, float l_eucl, int iterations,
+                      void *workspace, size_t workspace_bytes, void *stream) {
+  if (!spacing) return fail(FG_EINVAL, "spacing must have 3 entries");
+  return run_single(
+      spec3d(image, mask, out, C, Cm, D, H, W, spacing, FG_OP_UNSIGNED, iterations, v, l_grad, l_eucl, 0.f),
+      workspace, workspace_bytes, stream);
+}
+
+int fg_signed_geodesic2d_f32(const float *image, const float *mask, float *out, int C, int Cm, int H, int W,
+                             float v, float l_grad, float l_eucl, int iterations, void *workspace,
+                             size_t workspace_bytes, void *stream) {
+  return run_single(spec2d(image, mask, out, C, Cm, H, W, FG_OP_SIGNED, iterations, v, l_grad, l_eucl, 0.f),
+                    workspace, workspace_bytes, stream);
+}
+
+int fg_signed_geodesic3d_f32(const float *image, const float *mask, float *out, int C, int Cm, int D, int H,
+                             int W, const float spacing[3], float v, float l_grad, float l_eucl,
+                             int iterations, void *workspace, size_t workspace_bytes, void *stream) {
+  if (!spacing) return fail(FG_EINVAL, "spacing must have 3 entries");
+  return run_single(
+      spec3d(image, mask, out, C, Cm, D, H, W, spacing, FG_OP_SIGNED, iterations, v, l_grad, l_eucl, 0.f),
+      workspace, workspace_bytes, stream);
+}
+
+int fg_gsf2d_f32(const float *image, const float *mask, float *out, int C, int Cm, int H, int W, float theta,
+                 float v, float lambda, int iterations, void *workspace, size_t workspace_bytes,
+                 void *stream) {
+  return run_single(spec2d(image, mask, out, C, Cm, H, W, FG_OP_GSF, iterations, v, lambda, 0.f, theta),
+                    workspace, workspace_bytes, stream);
+}
+
+int fg_gsf3d_f32(const float *image, const float *mask, float *out, int C, int Cm, int D, int H, int W,
+                 float theta, const float spacing[3], float v, float lambda, int iterations, void *workspace,
+                 size_t workspace_bytes, void *stream) {
+  if (!spacing) return fail(FG_EINVAL, "spacing must have 3 entries");
+  return run_single(
+      spec3d(image, mask, out, C, Cm, D, H, W, spacing, FG_OP_GSF, iterations, v, lambda, 0.f, theta),
+      workspace, workspace_bytes, stream);
+}
+
+static int to_specs(const fg_instance *inst, int n, std::vector<OpSpec> &ops) {
+  if (n < 0 || (n > 0 && !inst)) return fail(FG_EINVAL, "bad instance array");
+  for (int i = 0; i < n; ++i) {
+    const fg_instance &f = inst[i];
+    if (f.is2d && f.D != 1) return fail(FG_EINVAL, "2D instance must have D == 1");
+    if (f.is2d)
+      ops.push_back(spec2d(f.image, f.mask, f.out, f.C, f.Cm, f.H, f.W, f.op, f.iterations, f.v, f.l_grad,
+                           f.l_eucl, f.theta));
+    else
+      ops.push_back(spec3d(f.image, f.mask, f.out, f.C, f.Cm, f.D, f.H, f.W, f.spacing, f.op, f.iterations,
+                           f.v, f.l_grad, f.l_eucl, f.theta));
+  }
+  return FG_OK;
+}
+
+size_t fg_batch_workspace_bytes(const fg_instance *inst, int n) {
+  std::vector<OpSpec> ops;
+  if (to_specs(inst, n, ops)) return 0;
+  return workspace_for(ops, current_halo());
+}
+
+int fg_geodesic_batch_f32(const fg_instance *inst, int n, void *workspace, size_t workspace_bytes,
+                          void *stream) {
+  std::vector<OpSpec> ops;
+  int rc = to_specs(inst, n, ops);
+  if (rc) return rc;
+  if (ops.empty()) return FG_OK;
+  return run_ops(ops, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+int fg_tuning_set(const fg_tuning *t) {
+  if (!t) return fail(FG_EINVAL, "null tuning");
+  if (t->engine < 0 || t->engine > 1) return fail(FG_EINVAL, "engine must be 0 or 1");
+  if (t->halo < 1 || t->halo > kMaxK) return fail(FG_EINVAL, "halo must be in [1, 16]");
+  if (t->max_groups < 0) return fail(FG_EINVAL, "max_groups must be >= 0");
+  g_tune = *t;
+  return FG_OK;
+}
+
+int fg_tuning_get(fg_tuning *t) {
+  if (!t) return fail(FG_EINVAL, "null tuning");
+  *t = g_tune;
+  return FG_OK;
+}
+
+int fg_profile_enable(int on) {
+  g_prof_on = on != 0;
+  return FG_OK;
+}
+
+int fg_profile_collect(int *launches, double *total_ms, double *alg_bytes) {
+  double ms = 0, bytes = 0;
+  int n = 0, rc = FG_OK;
+  for (auto &p : g_prof) {
+    float t = 0;
+    hipError_t e = hipEventSynchronize(p.stop);
+    if (e == hipSuccess) e = hipEventElapsedTime(&t, p.start, p.stop);
+    if (e != hipSuccess && rc == FG_OK) rc = fail(FG_EHIP, std::string("profile events: ") + hipGetErrorString(e));
+    (void)hipEventDestroy(p.start);
+    (void)hipEventDestroy(p.stop);
+    ms += t;
+    bytes += p.bytes;
+    ++n;
+  }
+  g_prof.clear();
+  if (launches) *launches = n;
+  if (total_ms) *total_ms = ms;
+  if (alg_bytes) *alg_bytes = bytes;
+  return rc;
+}
+
+int fg_status(const void *workspace, void *stream) {
+  if (!workspace) return fail(FG_EINVAL, "null workspace");
+  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+  if (e != hipSuccess) return fail(FG_EHIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
+  int st = 0;
+  e = hipMemcpy(&st, workspace, sizeof(int), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail(FG_EHIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+  if (st == FG_EWATCHDOG) return fail(FG_EWATCHDOG, "a tile's halo wait expired (watchdog)");
+  return FG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,156 @@
+/*
+ * fastgeodis_hip.h -- C ABI of the MI355X (gfx950) raster-scan generalised
+ * geodesic distance transform.
+ *
+ * Plain pointers and sizes only.  All pointers are device pointers on the
+ * current HIP device; `stream` is a hipStream_t passed as void* (NULL = the
+ * null stream).  Calls are asynchronous on `stream` and allocate nothing:
+ * scratch memory comes from the caller's `workspace` (query its size with
+ * fg_workspace_bytes).  The library keeps no mutable global state, so calls
+ * from several host threads on different streams may overlap, each with its
+ * own workspace.
+ *
+ * Layouts (batch 1, C-contiguous float32):
+ *   image (C, [D,] H, W), mask (Cm, [D,] H, W), out (Cm, [D,] H, W).
+ * Output channel 0 holds the transform; channels >= 1 hold the same
+ * elementwise function of the mask that the reference returns for them
+ * (reference fastgeodis_cpu.cpp:174, 356 initialise every channel, the passes
+ * touch channel 0 only).
+ *
+ * Return codes map onto the reference's Python exceptions:
+ *   FG_OK            0
+ *   FG_EINVAL       -1  invalid argument        -> ValueError
+ *   FG_EHIP         -2  HIP runtime error       -> RuntimeError
+ *   FG_EWATCHDOG    -3  a bounded device wait expired (a tile never arrived)
+ *                       -> RuntimeError
+ *   FG_EWORKSPACE   -4  workspace too small     -> RuntimeError
+ * fg_last_error() returns a thread-local message for the last failure.
+ */
+#ifndef FASTGEODIS_HIP_H
+#define FASTGEODIS_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FG_OK 0
+#define FG_EINVAL (-1)
+#define FG_EHIP (-2)
+#define FG_EWATCHDOG (-3)
+#define FG_EWORKSPACE (-4)
+
+/* Operation kinds for fg_workspace_bytes and fg_instance.op. */
+#define FG_OP_UNSIGNED 0
+#define FG_OP_SIGNED 1
+#define FG_OP_GSF 2
+
+/* ABI version of this header (bumped on any incompatible change). */
+int fg_abi_version(void);
+
+/* Thread-local text of the last error returned on this host thread. */
+const char *fg_last_error(void);
+
+/* Bytes of device workspace one call of kind `op` needs for an image of C
+ * channels over D*H*W voxels (D <= 1 selects 2D).  Depends on the tuning
+ * (fg_tuning_set) of the calling host thread. */
+size_t fg_workspace_bytes(int op, int C, int D, int H, int W);
+
+/* Synchronises `stream` and reads the status word at the start of
+ * `workspace` that the last call on it left: FG_OK, or FG_EWATCHDOG when a
+ * bounded device-side wait expired (the results of that call are invalid). */
+int fg_status(const void *workspace, void *stream);
+
+/* Replaces generalised_geodesic2d_cuda (reference fastgeodis.h:38-44,
+ * fastgeodis_cuda.cu:278-318): out = transform of fl(v*mask) after
+ * `iterations` iterations of (row pass pair, column pass pair).
+ * l_grad = lambda and l_eucl = 1 - lambda as the reference's Python wrapper
+ * passes them (FastGeodis/__init__.py:63-65). */
+int fg_geodesic2d_f32(const float *image, const float *mask, float *out, int C,
+                      int Cm, int H, int W, float v, float l_grad,
+                      float l_eucl, int iterations, void *workspace,
+                      size_t workspace_bytes, void *stream);
+
+/* Replaces generalised_geodesic3d_cuda (reference fastgeodis.h:46-54,
+ * fastgeodis_cuda.cu:587-656).  spacing = (s_D, s_H, s_W), unsquared; it is
+ * squared in float and the per-pass step tables are built in the reference's
+ * per-pass summation order (fastgeodis_cpu.cpp:220-231, 353, 365-397). */
+int fg_geodesic3d_f32(const float *image, const float *mask, float *out, int C,
+                      int Cm, int D, int H, int W, const float spacing[3],
+                      float v, float l_grad, float l_eucl, int iterations,
+                      void *workspace, size_t workspace_bytes, void *stream);
+
+/* Replaces signed_generalised_geodesic2d/3d (reference fastgeodis.cpp:241-258):
+ * out = D(mask) - D(1 - mask).  Both transforms run in the same launches. */
+int fg_signed_geodesic2d_f32(const float *image, const float *mask, float *out,
+                             int C, int Cm, int H, int W, float v,
+                             float l_grad, float l_eucl, int iterations,
+                             void *workspace, size_t workspace_bytes,
+                             void *stream);
+int fg_signed_geodesic3d_f32(const float *image, const float *mask, float *out,
+                             int C, int Cm, int D, int H, int W,
+                             const float spacing[3], float v, float l_grad,
+                             float l_eucl, int iterations, void *workspace,
+                             size_t workspace_bytes, void *stream);
+
+/* Replaces GSF2d/GSF3d (reference fastgeodis.cpp:319-343).  `lambda` is the
+ * raw lamb; l_eucl = 1 - lambda is formed in float as the reference does. */
+int fg_gsf2d_f32(const float *image, const float *mask, float *out, int C,
+                 int Cm, int H, int W, float theta, float v, float lambda,
+                 int iterations, void *workspace, size_t workspace_bytes,
+                 void *stream);
+int fg_gsf3d_f32(const float *image, const float *mask, float *out, int C,
+                 int Cm, int D, int H, int W, float theta,
+                 const float spacing[3], float v, float lambda, int iterations,
+                 void *workspace, size_t workspace_bytes, void *stream);
+
+/* ---- batched form (no counterpart in the reference, which accepts batch 1
+ * only, common.h:89-95): many instances of uneven sizes, 2D (D = 1) and 3D
+ * mixed, in the same persistent launches.  Each instance's result equals the
+ * single-instance call with the same arguments bit for bit. */
+typedef struct fg_instance {
+  const float *image; /* (C, D, H, W) */
+  const float *mask;  /* (Cm, D, H, W) */
+  float *out;         /* (Cm, D, H, W) */
+  int C, Cm, D, H, W;
+  int is2d;           /* 1: 2D semantics (D must be 1, spacing ignored) */
+  int op;             /* FG_OP_UNSIGNED / FG_OP_SIGNED / FG_OP_GSF */
+  int iterations;
+  float spacing[3];   /* unsquared (s_D, s_H, s_W); 3D only */
+  float v;
+  float l_grad;       /* unsigned/signed: lambda; GSF: lambda */
+  float l_eucl;       /* unsigned/signed: 1 - lambda; GSF: ignored */
+  float theta;        /* GSF only */
+} fg_instance;
+
+size_t fg_batch_workspace_bytes(const fg_instance *inst, int n);
+int fg_geodesic_batch_f32(const fg_instance *inst, int n, void *workspace,
+                          size_t workspace_bytes, void *stream);
+
+/* Tile/launch parameters of the persistent sweep engine.  engine 0 is the
+ * one-launch-per-slice engine, 1 the persistent tiled engine (default).
+ * halo is the trapezoid depth K (slices per neighbour exchange).  Values are
+ * per host thread; fg_tuning_get reads them back. */
+typedef struct fg_tuning {
+  int engine;
+  int halo;
+  int max_groups; /* cap on persistent workgroups (0 = occupancy limit) */
+} fg_tuning;
+int fg_tuning_set(const fg_tuning *t);
+int fg_tuning_get(fg_tuning *t);
+
+/* Per-launch timing of the persistent sweep kernels, for measurement tools
+ * (no counterpart in the reference).  While enabled on a host thread, every
+ * tiled sweep launch it issues is bracketed by HIP events on its stream.
+ * fg_profile_collect waits for those events and returns the number of
+ * launches, their summed duration (ms) and their summed algorithmic bytes
+ * ((4*C + 8) bytes per voxel update), then clears the record. */
+int fg_profile_enable(int on);
+int fg_profile_collect(int *launches, double *total_ms, double *alg_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FASTGEODIS_HIP_H */

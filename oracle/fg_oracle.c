@@ -1,0 +1,274 @@
+/*
+ * fg_oracle.c -- CPU restatement of FastGeodis' raster-scan generalised
+ * geodesic distance transform (reference v1.0.5).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * only as the checker.  The product path (fastgeodis_amd) never calls it.
+ *
+ * Semantics followed (reference file:line):
+ *   - init  d = fl(v * mask) for every mask channel      fastgeodis_cpu.cpp:174, 356
+ *   - 2D pass pair (top-down then bottom-up along rows)  fastgeodis_cpu.cpp:53-162
+ *   - 2D driver: rows, then columns (via transpose)      fastgeodis_cpu.cpp:164-201
+ *   - 3D pass pair with per-pass 3x3 step table          fastgeodis_cpu.cpp:203-340
+ *   - 3D driver: spacing squared in float, D/H/W passes  fastgeodis_cpu.cpp:342-407
+ *   - l1distance |a-b|, channel sum from 0.0f            fastgeodis_cpu.cpp:38-41, 88-105
+ *   - signed = D(M) - D(1-M)                             fastgeodis.cpp:241-258
+ *   - GSF: three signed transforms, thresholds +-theta   fastgeodis.cpp:319-343
+ *
+ * The reference transposes tensors so that every sweep runs along the
+ * outermost spatial axis; transposes only move data, so this restatement
+ * indexes the native layout with explicit strides instead.  Each candidate is
+ * evaluated as (d_q + l_eucl*LD) + l_grad*l with two float roundings and no
+ * fused multiply-add (build with -ffp-contract=off), and the running minimum
+ * keeps the current value unless the candidate is strictly smaller, which is
+ * std::min's rule.  Neighbours are visited in the reference order (first
+ * in-plane axis outer, second inner).
+ *
+ * The per-slice loops are OpenMP-parallel like the reference (one parallel
+ * region per row or plane), so the same code serves as a CPU baseline.
+ */
+#include <math.h>
+#include <stddef.h>
+#include <stdlib.h>
+#include <string.h>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* std::min(a, b): returns b only when b < a (fastgeodis_cpu.cpp:110). */
+static inline float ref_min(float a, float b) { return (b < a) ? b : a; }
+
+/*
+ * One pass pair (forward sweep then backward sweep) along axis A of one
+ * distance channel.  Slices along A are updated in order; every element of
+ * slice a is recomputed from the already-updated slice a-1 (forward) or a+1
+ * (backward) over the 3x3 neighbourhood spanned by in-plane axes (B, E).
+ * Out-of-range neighbours are skipped.  ld[i*3+j] is the step length for
+ * offset (i-1, j-1) in (B, E).  Mirrors fastgeodis_cpu.cpp:234-339 (3D) and,
+ * with nB == 1 and ld row 1 = {sqrt2, 1, sqrt2}, fastgeodis_cpu.cpp:70-161 (2D).
+ */
+static void pass_pair(const float *img, float *d, int C, ptrdiff_t cs,
+                      int nA, ptrdiff_t sA, int nB, ptrdiff_t sB, int nE,
+                      ptrdiff_t sE, const float ld[9], float l_grad,
+                      float l_eucl) {
+  for (int dir = 0; dir < 2; ++dir) {
+    const int step = dir == 0 ? 1 : -1;
+    const int first = dir == 0 ? 1 : nA - 2;
+    for (int a = first; a >= 0 && a < nA; a += step) {
+      const int ap = a - step;
+#pragma omp parallel for collapse(2) schedule(static)
+      for (int b = 0; b < nB; ++b) {
+        for (int e = 0; e < nE; ++e) {
+          const ptrdiff_t p = (ptrdiff_t)a * sA + (ptrdiff_t)b * sB + (ptrdiff_t)e * sE;
+          float new_dist = d[p];
+          for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 3; ++j) {
+              const int bq = b + i - 1, eq = e + j - 1;
+              if (bq < 0 || bq >= nB || eq < 0 || eq >= nE) continue;
+              const ptrdiff_t q = (ptrdiff_t)ap * sA + (ptrdiff_t)bq * sB + (ptrdiff_t)eq * sE;
+              float l_dist = 0.0f;
+              if (C == 1) {
+                l_dist = fabsf(img[p] - img[q]);
+              } else {
+                for (int c = 0; c < C; ++c)
+                  l_dist += fabsf(img[(ptrdiff_t)c * cs + p] - img[(ptrdiff_t)c * cs + q]);
+              }
+              const float cur = d[q] + l_eucl * ld[i * 3 + j] + l_grad * l_dist;
+              new_dist = ref_min(new_dist, cur);
+            }
+          }
+          d[p] = new_dist;
+        }
+      }
+    }
+  }
+}
+
+/* Step table of one 3D pass: sqrt((s0 + |i-1|*s1) + |j-1|*s2), summed in this
+ * order in float (fastgeodis_cpu.cpp:220-231).  s* are squared spacings. */
+static void table3d(float s0, float s1, float s2, float ld[9]) {
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) {
+      float v = s0;
+      v += (float)abs(i - 1) * s1;
+      v += (float)abs(j - 1) * s2;
+      ld[i * 3 + j] = sqrtf(v);
+    }
+  }
+}
+
+static void init_dist(const float *mask, float *out, size_t n, float v) {
+  for (size_t i = 0; i < n; ++i) out[i] = v * mask[i];
+}
+
+/* Runs the 2D iterations on distance channel 0 (fastgeodis_cpu.cpp:177-198). */
+static void transform2d(const float *img, float *d, int C, int H, int W,
+                        float l_grad, float l_eucl, int iters) {
+  const float r2 = sqrtf(2.0f);
+  const float ld[9] = {0.f, 0.f, 0.f, r2, 1.0f, r2, 0.f, 0.f, 0.f};
+  const ptrdiff_t cs = (ptrdiff_t)H * W;
+  for (int it = 0; it < iters; ++it) {
+    /* rows: sweep along H, neighbours along W */
+    pass_pair(img, d, C, cs, H, W, 1, 0, W, 1, ld, l_grad, l_eucl);
+    /* columns: sweep along W, neighbours along H */
+    pass_pair(img, d, C, cs, W, 1, 1, 0, H, W, ld, l_grad, l_eucl);
+  }
+}
+
+/* Runs the 3D iterations on distance channel 0 (fastgeodis_cpu.cpp:359-404).
+ * sq[] holds the squared spacings (D, H, W). */
+static void transform3d(const float *img, float *d, int C, int D, int H, int W,
+                        const float sq[3], float l_grad, float l_eucl,
+                        int iters) {
+  float ldD[9], ldH[9], ldW[9];
+  table3d(sq[0], sq[1], sq[2], ldD); /* pass along D: in-plane (H, W) */
+  table3d(sq[1], sq[0], sq[2], ldH); /* pass along H: in-plane (D, W) */
+  table3d(sq[2], sq[1], sq[0], ldW); /* pass along W: in-plane (H, D) */
+  const ptrdiff_t sD = (ptrdiff_t)H * W, sH = W, sW = 1;
+  const ptrdiff_t cs = (ptrdiff_t)D * H * W;
+  for (int it = 0; it < iters; ++it) {
+    pass_pair(img, d, C, cs, D, sD, H, sH, W, sW, ldD, l_grad, l_eucl);
+    pass_pair(img, d, C, cs, H, sH, D, sD, W, sW, ldH, l_grad, l_eucl);
+    pass_pair(img, d, C, cs, W, sW, H, sH, D, sD, ldW, l_grad, l_eucl);
+  }
+}
+
+/* ---- public entry points (ctypes) ---------------------------------------
+ * Shapes: image (1, C, [D,] H, W), mask/out (1, Cm, [D,] H, W), contiguous.
+ * Output channel 0 is transformed; channels >= 1 hold fl(v * mask). */
+
+int fgo_geodesic2d(const float *img, const float *mask, float *out, int C,
+                   int Cm, int H, int W, float v, float l_grad, float l_eucl,
+                   int iters) {
+  if (C < 1 || Cm < 1 || H < 1 || W < 1) return -1;
+  init_dist(mask, out, (size_t)Cm * H * W, v);
+  transform2d(img, out, C, H, W, l_grad, l_eucl, iters);
+  return 0;
+}
+
+int fgo_geodesic3d(const float *img, const float *mask, float *out, int C,
+                   int Cm, int D, int H, int W, const float spacing[3], float v,
+                   float l_grad, float l_eucl, int iters) {
+  if (C < 1 || Cm < 1 || D < 1 || H < 1 || W < 1) return -1;
+  float sq[3];
+  for (int k = 0; k < 3; ++k) sq[k] = spacing[k] * spacing[k];
+  init_dist(mask, out, (size_t)Cm * D * H * W, v);
+  transform3d(img, out, C, D, H, W, sq, l_grad, l_eucl, iters);
+  return 0;
+}
+
+/* spatial dims packed as dims[0..nd-1]; D = 1 for 2D. */
+static int unsigned_nd(const float *img, const float *mask, float *out, int C,
+                       int Cm, int nd, const int *dims, const float *spacing,
+                       float v, float l_grad, float l_eucl, int iters) {
+  if (nd == 2)
+    return fgo_geodesic2d(img, mask, out, C, Cm, dims[0], dims[1], v, l_grad,
+                          l_eucl, iters);
+  return fgo_geodesic3d(img, mask, out, C, Cm, dims[0], dims[1], dims[2],
+                        spacing, v, l_grad, l_eucl, iters);
+}
+
+/* signed = D(M) - D(1 - M)  (fastgeodis.cpp:241-258) */
+static int signed_nd(const float *img, const float *mask, float *out, int C,
+                     int Cm, int nd, const int *dims, const float *spacing,
+                     float v, float l_grad, float l_eucl, int iters) {
+  size_t n = (size_t)Cm;
+  for (int k = 0; k < nd; ++k) n *= (size_t)dims[k];
+  float *inv = (float *)malloc(n * sizeof(float));
+  float *db = (float *)malloc(n * sizeof(float));
+  if (!inv || !db) {
+    free(inv);
+    free(db);
+    return -2;
+  }
+  for (size_t i = 0; i < n; ++i) inv[i] = 1.0f - mask[i];
+  int rc = unsigned_nd(img, mask, out, C, Cm, nd, dims, spacing, v, l_grad,
+                       l_eucl, iters);
+  if (rc == 0)
+    rc = unsigned_nd(img, inv, db, C, Cm, nd, dims, spacing, v, l_grad, l_eucl,
+                     iters);
+  if (rc == 0)
+    for (size_t i = 0; i < n; ++i) out[i] = out[i] - db[i];
+  free(inv);
+  free(db);
+  return rc;
+}
+
+/* GSF (fastgeodis.cpp:319-343): l_eucl = 1 - lambda in float. */
+static int gsf_nd(const float *img, const float *mask, float *out, int C,
+                  int Cm, int nd, const int *dims, const float *spacing,
+                  float theta, float v, float lambda, int iters) {
+  size_t n = (size_t)Cm;
+  for (int k = 0; k < nd; ++k) n *= (size_t)dims[k];
+  const float l_eucl = 1 - lambda;
+  float *ds = (float *)malloc(n * sizeof(float));
+  float *m = (float *)malloc(n * sizeof(float));
+  float *dd = (float *)malloc(n * sizeof(float));
+  if (!ds || !m || !dd) {
+    free(ds);
+    free(m);
+    free(dd);
+    return -2;
+  }
+  int rc = signed_nd(img, mask, ds, C, Cm, nd, dims, spacing, v, lambda,
+                     l_eucl, iters);
+  if (rc == 0) {
+    /* Dd_Md = -signed(1 - Md), Md = (Ds > theta) */
+    for (size_t i = 0; i < n; ++i) m[i] = 1.0f - ((ds[i] > theta) ? 1.0f : 0.0f);
+    rc = signed_nd(img, m, dd, C, Cm, nd, dims, spacing, v, lambda, l_eucl,
+                   iters);
+  }
+  if (rc == 0) {
+    /* De_Me = signed(Me), Me = (Ds > -theta) */
+    for (size_t i = 0; i < n; ++i) m[i] = (ds[i] > -theta) ? 1.0f : 0.0f;
+    rc = signed_nd(img, m, out, C, Cm, nd, dims, spacing, v, lambda, l_eucl,
+                   iters);
+  }
+  if (rc == 0)
+    for (size_t i = 0; i < n; ++i) out[i] = -dd[i] + out[i];
+  free(ds);
+  free(m);
+  free(dd);
+  return rc;
+}
+
+int fgo_signed_geodesic2d(const float *img, const float *mask, float *out,
+                          int C, int Cm, int H, int W, float v, float l_grad,
+                          float l_eucl, int iters) {
+  const int dims[2] = {H, W};
+  return signed_nd(img, mask, out, C, Cm, 2, dims, NULL, v, l_grad, l_eucl,
+                   iters);
+}
+
+int fgo_signed_geodesic3d(const float *img, const float *mask, float *out,
+                          int C, int Cm, int D, int H, int W,
+                          const float spacing[3], float v, float l_grad,
+                          float l_eucl, int iters) {
+  const int dims[3] = {D, H, W};
+  return signed_nd(img, mask, out, C, Cm, 3, dims, spacing, v, l_grad, l_eucl,
+                   iters);
+}
+
+int fgo_gsf2d(const float *img, const float *mask, float *out, int C, int Cm,
+              int H, int W, float theta, float v, float lambda, int iters) {
+  const int dims[2] = {H, W};
+  return gsf_nd(img, mask, out, C, Cm, 2, dims, NULL, theta, v, lambda, iters);
+}
+
+int fgo_gsf3d(const float *img, const float *mask, float *out, int C, int Cm,
+              int D, int H, int W, float theta, const float spacing[3],
+              float v, float lambda, int iters) {
+  const int dims[3] = {D, H, W};
+  return gsf_nd(img, mask, out, C, Cm, 3, dims, spacing, theta, v, lambda,
+                iters);
+}
+
+int fgo_num_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}

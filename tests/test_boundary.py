@@ -1,0 +1,159 @@
+"""Host-side tests of the drop-in boundary (no GPU compute).
+
+* the C-ABI library loads and exports every function include/fastgeodis_hip.h
+  declares; argument validation returns the documented codes;
+* the torch glue raises the reference's exception types in the reference's
+  order (FastGeodis/common.h:53-120, fastgeodis.cpp:44-146;
+  tests/test_fastgeodis.py:46-91, 212-252);
+* the Python signatures match the reference wrappers.
+"""
+import ctypes
+import inspect
+import os
+import re
+
+import pytest
+import torch
+
+import fastgeodis_amd as FG
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "fastgeodis_hip.h")
+LIB = os.path.join(ROOT, "fastgeodis_amd", "lib", "libfastgeodis_hip.so")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fg_\w+)\s*\(", text)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return ctypes.CDLL(LIB)
+
+
+def test_library_exports_every_declared_symbol(lib):
+    names = declared_functions()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(lib, n), f"{n} declared in the header but not exported"
+
+
+def test_abi_version_and_errors(lib):
+    lib.fg_abi_version.restype = ctypes.c_int
+    assert lib.fg_abi_version() == 1
+    lib.fg_last_error.restype = ctypes.c_char_p
+    f = lib.fg_geodesic2d_f32
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_int] * 4 + [ctypes.c_float] * 3 + \
+        [ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p]
+    rc = f(None, None, None, 1, 1, 8, 8, 1e10, 1.0, 0.0, 2, None, 0, None)
+    assert rc == -1
+    assert b"null" in lib.fg_last_error()
+
+
+def test_workspace_query(lib):
+    q = lib.fg_workspace_bytes
+    q.restype = ctypes.c_size_t
+    q.argtypes = [ctypes.c_int] * 5
+    small = q(0, 1, 1, 64, 64)
+    big = q(0, 1, 64, 64, 64)
+    gsf = q(2, 3, 64, 64, 64)
+    assert 0 < small < big < gsf
+    # the image transpose and the transposed distances dominate: >= (C+1)*N floats
+    assert big >= 2 * 64 ** 3 * 4
+
+
+def test_tuning_roundtrip():
+    old = FG.get_tuning()
+    try:
+        FG.set_tuning(0, 3, 17)
+        assert FG.get_tuning() == {"engine": 0, "halo": 3, "max_groups": 17}
+        with pytest.raises(ValueError):
+            FG.set_tuning(2, 4, 0)
+        with pytest.raises(ValueError):
+            FG.set_tuning(1, 0, 0)
+        with pytest.raises(ValueError):
+            FG.set_tuning(1, 17, 0)
+    finally:
+        FG.set_tuning(**old)
+
+
+def test_signatures_match_reference_wrappers():
+    sig = {n: list(inspect.signature(getattr(FG, n)).parameters) for n in (
+        "generalised_geodesic2d", "generalised_geodesic3d", "signed_generalised_geodesic2d",
+        "signed_generalised_geodesic3d", "GSF2d", "GSF3d")}
+    assert sig["generalised_geodesic2d"] == ["image", "softmask", "v", "lamb", "iter"]
+    assert sig["generalised_geodesic3d"] == ["image", "softmask", "spacing", "v", "lamb", "iter"]
+    assert sig["signed_generalised_geodesic2d"] == ["image", "softmask", "v", "lamb", "iter"]
+    assert sig["signed_generalised_geodesic3d"] == ["image", "softmask", "spacing", "v", "lamb", "iter"]
+    assert sig["GSF2d"] == ["image", "softmask", "theta", "v", "lamb", "iter"]
+    assert sig["GSF3d"] == ["image", "softmask", "theta", "spacing", "v", "lamb", "iter"]
+    assert inspect.signature(FG.generalised_geodesic2d).parameters["iter"].default == 2
+    assert inspect.signature(FG.generalised_geodesic3d).parameters["iter"].default == 4
+
+
+FUNCS = [
+    (2, lambda i, m: FG.generalised_geodesic2d(i, m, 1e10, 1.0, 2)),
+    (3, lambda i, m: FG.generalised_geodesic3d(i, m, [1.0, 1.0, 1.0], 1e10, 1.0, 2)),
+    (2, lambda i, m: FG.signed_generalised_geodesic2d(i, m, 1e10, 1.0, 2)),
+    (3, lambda i, m: FG.signed_generalised_geodesic3d(i, m, [1.0, 1.0, 1.0], 1e10, 1.0, 2)),
+    (2, lambda i, m: FG.GSF2d(i, m, 0.0, 1e10, 1.0, 2)),
+    (3, lambda i, m: FG.GSF3d(i, m, 0.0, [1.0, 1.0, 1.0], 1e10, 1.0, 2)),
+]
+
+
+@pytest.mark.parametrize("nd,fn", FUNCS)
+@pytest.mark.parametrize("base", [16, 32])
+def test_ill_shape_raises_value_error(nd, fn, base):
+    shape = [1, 1] + [base] * nd
+    bad_batch = [2] + shape[1:]
+    with pytest.raises(ValueError):
+        fn(torch.rand(shape), torch.rand(bad_batch))
+    with pytest.raises(ValueError):
+        fn(torch.rand(bad_batch), torch.rand(bad_batch))
+    mism = list(shape)
+    mism[-1] = 12
+    with pytest.raises(ValueError):
+        fn(torch.rand(mism), torch.rand(shape))
+    if nd == 2:
+        with pytest.raises(ValueError):
+            fn(torch.rand(shape + [16]), torch.rand(shape + [16]))
+
+
+@pytest.mark.parametrize("fn", [
+    lambda i, m, s: FG.generalised_geodesic3d(i, m, s, 1e10, 1.0, 2),
+    lambda i, m, s: FG.signed_generalised_geodesic3d(i, m, s, 1e10, 1.0, 2),
+    lambda i, m, s: FG.GSF3d(i, m, 0.0, s, 1e10, 1.0, 2),
+])
+def test_ill_spacing_raises_value_error(fn):
+    x = torch.zeros(1, 1, 16, 16, 16)
+    with pytest.raises(ValueError):
+        fn(x, x, [1.0, 1.0])
+
+
+def test_channel_counts_need_not_match():
+    """The reference compares spatial dims only (common.h:53-71): C and Cm may
+    differ; the call proceeds past validation (and here stops at the device)."""
+    img = torch.rand(1, 3, 8, 8)
+    mask = torch.rand(1, 2, 8, 8)
+    if torch.cuda.is_available():
+        out = FG.generalised_geodesic2d(img, mask, 1e10, 1.0, 2)
+        assert out.shape == mask.shape
+    else:
+        with pytest.raises(RuntimeError):
+            FG.generalised_geodesic2d(img, mask, 1e10, 1.0, 2)
+
+
+@pytest.mark.skipif(torch.cuda.is_available(), reason="checks the no-GPU error path")
+def test_cpu_tensors_without_gpu_raise_runtime_error():
+    x = torch.zeros(1, 1, 8, 8)
+    with pytest.raises(RuntimeError, match="ROCm GPU"):
+        FG.generalised_geodesic2d(x, x, 1e10, 1.0, 2)
+
+
+def test_native_libraries_are_in_tree():
+    for p in FG.native_library_paths():
+        assert os.path.isfile(p)
+        assert os.path.commonpath([p, ROOT]) == ROOT
