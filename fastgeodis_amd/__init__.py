@@ -176,15 +176,16 @@ def geodesic_batch(instances: Sequence[Instance]) -> List[torch.Tensor]:
     )
 
 
-def set_tuning(engine: int = 1, halo: int = 4, max_groups: int = 0) -> None:
-    """Engine (0 per-slice, 1 persistent tiled), halo depth K and workgroup
-    cap used by subsequent calls on this host thread."""
-    FastGeodisCpp.set_tuning(int(engine), int(halo), int(max_groups))
+def set_tuning(engine: int = 1, halo: int = 4, max_groups: int = 0, shape: int = 0) -> None:
+    """Engine (0 per-slice, 1 persistent tiled), halo depth K, workgroup cap
+    (0 = occupancy limit) and 3D plane workgroup shape (0 auto, 1 = 4 waves x
+    8 rows, 2 = 8 x 4, 3 = 16 x 2) used by later calls on this host thread."""
+    FastGeodisCpp.set_tuning(int(engine), int(halo), int(max_groups), int(shape))
 
 
 def get_tuning():
-    e, h, g = FastGeodisCpp.get_tuning()
-    return {"engine": e, "halo": h, "max_groups": g}
+    e, h, g, s = FastGeodisCpp.get_tuning()
+    return {"engine": e, "halo": h, "max_groups": g, "shape": s}
 
 
 def native_library_paths() -> List[str]:

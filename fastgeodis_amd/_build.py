@@ -33,7 +33,6 @@ HIP_FLAGS = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-command-line-argument",
 ]
 
-_SCALAR_STORE = re.compile(r"^\s*(s_store_|s_buffer_store|s_scratch_store|s_atomic|s_buffer_atomic|s_dcache_wb|s_dcache_discard)", re.M)
 
 
 def _run(cmd, **kw):
@@ -46,14 +45,16 @@ def _run(cmd, **kw):
 
 def check_device_asm(src: str) -> None:
     """Compile the device side to assembly and refuse scalar-cache writes."""
+    from ._asm_guard import scalar_cache_writes
+
     asm = os.path.join(LIB_DIR, "fg_kernels.s")
     _run([HIPCC, *HIP_FLAGS, "-I", INCLUDE, "--cuda-device-only", "-S", src, "-o", asm])
     with open(asm) as f:
         text = f.read()
-    bad = _SCALAR_STORE.findall(text)
     os.remove(asm)
+    bad = scalar_cache_writes(text)
     if bad:
-        raise RuntimeError(f"scalar-cache store instructions in device code: {sorted(set(bad))}")
+        raise RuntimeError(f"scalar-cache write instructions in device code: {bad}")
 
 
 def build_hip(verbose: bool = False) -> str:

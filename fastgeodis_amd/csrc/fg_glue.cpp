@@ -126,7 +126,7 @@ torch::Tensor run_device(const Call &c, const torch::Tensor &image_in, const tor
     W = (int)image.size(4);
   }
   if (out.numel() == 0 || image.numel() == 0) return out;
-  const size_t wsb = fg_workspace_bytes(c.op, C, c.nd == 2 ? 1 : D, H, W);
+  const size_t wsb = fg_workspace_bytes(c.op, C, c.nd == 2 ? 0 : D, H, W);
   torch::Tensor ws = torch::empty({(int64_t)wsb}, image.options().dtype(torch::kUInt8));
   void *stream = (void *)c10::hip::getCurrentHIPStream(image.device().index()).stream();
   const float *ip = image.data_ptr<float>();
@@ -290,15 +290,15 @@ std::vector<torch::Tensor> geodesic_batch(const std::vector<torch::Tensor> &imag
   return outs;
 }
 
-void set_tuning(int engine, int halo, int max_groups) {
-  fg_tuning t{engine, halo, max_groups};
+void set_tuning(int engine, int halo, int max_groups, int shape) {
+  fg_tuning t{engine, halo, max_groups, shape};
   raise_for(fg_tuning_set(&t));
 }
 
 std::vector<int64_t> get_tuning() {
   fg_tuning t{};
   raise_for(fg_tuning_get(&t));
-  return {t.engine, t.halo, t.max_groups};
+  return {t.engine, t.halo, t.max_groups, t.shape};
 }
 
 void profile_enable(bool on) { raise_for(fg_profile_enable(on ? 1 : 0)); }

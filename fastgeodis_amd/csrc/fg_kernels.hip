@@ -87,19 +87,17 @@ struct XposeDesc {  // out[c][w][h][d] = in[c][d][h][w]
 // Device helpers
 // ---------------------------------------------------------------------------
 
-// Lane l receives lane l-1 (lane 0 receives 0).
-__device__ __forceinline__ float from_left(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x138, 0xF, 0xF, false));
-}
-// Lane l receives lane l+1 (lane 63 receives 0).
-__device__ __forceinline__ float from_right(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x130, 0xF, 0xF, false));
-}
-template <int J>
+// Lane l receives lane l-1 (J == 0) or lane l+1 (J == 2); the edge lane
+// receives 0.  TAG only makes each call site distinct: with bound_ctrl set
+// the "old" operand is never used, and distinct call sites keep the compiler
+// from merging shifts of one value, so every shift folds into the single
+// VALU instruction that consumes it (v_add_f32_dpp / v_sub_f32_dpp).
+template <int J, int TAG>
 __device__ __forceinline__ float shiftE(float x) {
-  if constexpr (J == 0) return from_left(x);
-  else if constexpr (J == 2) return from_right(x);
-  else return x;
+  if constexpr (J == 1) return x;
+  else
+    return __int_as_float(__builtin_amdgcn_update_dpp(TAG, __float_as_int(x), J == 0 ? 0x138 : 0x130, 0xF,
+                                                      0xF, true));
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
@@ -115,20 +113,42 @@ __device__ __forceinline__ unsigned long long pack(float v, unsigned tag) {
   return ((unsigned long long)tag << 32) | (unsigned long long)__float_as_uint(v);
 }
 
-// One candidate for the element at this lane from the front neighbour
-// (already shifted to this lane).
-template <int CT, bool LAMB1>
-__device__ __forceinline__ float candidate(float dq, const float (&Iq)[CT], const float (&In)[CT],
-                                           float e1, float lg) {
-  float l = fabsf(In[0] - Iq[0]);
-#pragma unroll
-  for (int c = 1; c < CT; ++c) l = l + fabsf(In[c] - Iq[c]);
+// sum over channels C0..CT-1 of |I_c(p) - I_c(q)|, accumulated left to right
+template <int I, int J, int C0, int CT>
+__device__ __forceinline__ float absdiff_sum(float acc, const float (&Iq)[CT], const float (&In)[CT]) {
+  if constexpr (C0 >= CT) {
+    return acc;
+  } else {
+    const float q = shiftE<J, 1 + C0 + I * 16 + J * 4>(Iq[C0]);
+    return absdiff_sum<I, J, C0 + 1, CT>(acc + fabsf(q - In[C0]), Iq, In);
+  }
+}
+
+// Candidate for the element at this lane from front row I (B offset I-1) and
+// lane offset J-1:  (d_q + e1) + l_grad * sum_c |I_c(p) - I_c(q)|.  The
+// channel sum starts from |I_0(p) - I_0(q)|, which equals the reference's
+// 0.0f + |..| bit for bit.
+template <int I, int J, int CT, bool LAMB1>
+__device__ __forceinline__ float candidate(float dq, const float (&Iq)[CT], const float (&In)[CT], float e1,
+                                           float lg) {
+  // |q - p| == |p - q| exactly; the shifted operand goes first so that the
+  // shift folds into v_sub_f32_dpp without commuting
+  const float l = absdiff_sum<I, J, 1, CT>(fabsf(shiftE<J, 1 + I * 16 + J * 4>(Iq[0]) - In[0]), Iq, In);
+  const float ds = shiftE<J, 64 + I * 4 + J>(dq);
   if constexpr (LAMB1) {
     // l_grad == 1 and l_eucl*LD == +0: fl(1*l) == l and dropping "+ 0" is exact here
-    return dq + l;
+    return ds + l;
   } else {
-    return (dq + e1) + lg * l;
+    return (ds + e1) + lg * l;
   }
+}
+
+template <int I, int CT, bool LAMB1>
+__device__ __forceinline__ void candidates_row(float (&c)[9], float dq, const float (&Iq)[CT],
+                                               const float (&In)[CT], const float (&e1)[9], float lg) {
+  c[I * 3 + 0] = candidate<I, 0, CT, LAMB1>(dq, Iq, In, e1[I * 3 + 0], lg);
+  c[I * 3 + 1] = candidate<I, 1, CT, LAMB1>(dq, Iq, In, e1[I * 3 + 1], lg);
+  c[I * 3 + 2] = candidate<I, 2, CT, LAMB1>(dq, Iq, In, e1[I * 3 + 2], lg);
 }
 
 // New value of one row element.  dq[i] / Iq[i][c]: front rows at B offset
@@ -138,19 +158,9 @@ __device__ __forceinline__ float relax(const float (&dq)[3], const float (&Iq)[3
                                        const float (&In)[CT], float dinit, const float (&e1)[9],
                                        float lg) {
   float c[9];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    float s0[CT], s1[CT], s2[CT];
-#pragma unroll
-    for (int ch = 0; ch < CT; ++ch) {
-      s0[ch] = shiftE<0>(Iq[i][ch]);
-      s1[ch] = Iq[i][ch];
-      s2[ch] = shiftE<2>(Iq[i][ch]);
-    }
-    c[i * 3 + 0] = candidate<CT, LAMB1>(shiftE<0>(dq[i]), s0, In, e1[i * 3 + 0], lg);
-    c[i * 3 + 1] = candidate<CT, LAMB1>(dq[i], s1, In, e1[i * 3 + 1], lg);
-    c[i * 3 + 2] = candidate<CT, LAMB1>(shiftE<2>(dq[i]), s2, In, e1[i * 3 + 2], lg);
-  }
+  candidates_row<0, CT, LAMB1>(c, dq[0], Iq[0], In, e1, lg);
+  candidates_row<1, CT, LAMB1>(c, dq[1], Iq[1], In, e1, lg);
+  candidates_row<2, CT, LAMB1>(c, dq[2], Iq[2], In, e1, lg);
   if constexpr (FAST) {
     float m = fminf(fminf(c[0], c[1]), c[2]);
     m = fminf(fminf(m, c[3]), c[4]);
@@ -169,28 +179,39 @@ __device__ __forceinline__ float relax(const float (&dq)[3], const float (&Iq)[3
 // Tiled persistent sweep kernel
 // ---------------------------------------------------------------------------
 
-template <int NW, int R, int CT>
-struct Front {
-  float d[R];           // front distance, own wave rows
-  float I[CT][R + 2];   // front image, rows -1..R relative to the wave
-};
+// Global-address-space views: loads and stores then use SGPR base + VGPR
+// offset addressing instead of flat addressing.
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) unsigned long long gu64;
+
+// Loads/stores at a uniform base plus a 32-bit per-lane byte offset, which
+// maps onto SGPR-base + VGPR-offset addressing.
+typedef __attribute__((address_space(1))) const char gcbyte;
+typedef __attribute__((address_space(1))) char gbyte;
+__device__ __forceinline__ float ld_at(const gfloat *base, unsigned byte_off) {
+  return *(const gfloat *)((gcbyte *)base + byte_off);
+}
+__device__ __forceinline__ void st_at(gfloat *base, unsigned byte_off, float v) {
+  *(gfloat *)((gbyte *)base + byte_off) = v;
+}
 
 template <int R, int CT>
-struct Slab {            // one prefetched slice
+struct Slab {  // one prefetched slice: image rows -1..R, distance rows 0..R-1
   float I[CT][R + 2];
   float d[R];
 };
 
-template <int NW, int R, int CT, bool LAMB1, bool FAST>
+// NW waves per workgroup, R window rows per wave (register rows), P slices
+// prefetched ahead, CT image channels.
+template <int NW, int R, int P, int CT, bool LAMB1, bool FAST>
 __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     const SweepDesc *__restrict__ descs, int ndesc, int K, unsigned tag_base,
-    unsigned long long *__restrict__ halo, int *__restrict__ status, int spin_limit) {
-  constexpr int WR = NW * R;
+    unsigned long long *__restrict__ halo_base, int *__restrict__ status, int spin_limit) {
   __shared__ float edge_first[2][NW][64];
   __shared__ float edge_last[2][NW][64];
 
   const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index: uniform
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
 
   // descriptor owning this tile (binary search on tile0)
@@ -200,8 +221,9 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     if (descs[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
   }
   const SweepDesc &Dd = descs[lo];
-  const float *__restrict__ img = Dd.img;
-  float *__restrict__ dist = Dd.d;
+  const gfloat *img = (const gfloat *)Dd.img;
+  gfloat *dist = (gfloat *)Dd.d;
+  gu64 *halo = (gu64 *)halo_base + Dd.halo0;
   const long long cs = Dd.cs, sA = Dd.sA, sB = Dd.sB;
   const int nA = Dd.nA, nB = Dd.nB, nE = Dd.nE, dir = Dd.dir;
   const float lg = Dd.lg;
@@ -217,146 +239,201 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   const int ew0 = (nTE > 1) ? e_own0 - K : -1;
   const int e = ew0 + lane;
   const bool lane_valid = (e >= 0) && (e < nE);
+  const unsigned ec = (unsigned)min(max(e, 0), nE - 1);  // clamped: loads never leave the row
+  const unsigned ecb = ec * 4u;                             // its byte offset within a row
   const bool lane_own = (e >= e_own0) && (e < e_own1);
   const bool pad_lanes = (ew0 < 0) || (ew0 + 63 >= nE);
-  const int own_w = e_own1 - e_own0;
-  const unsigned long long *halo_ro = halo;
-  unsigned long long *my_halo = halo + Dd.halo0 + (long long)lt * 2 * TB * TE;
+  const long long tile_slots = (long long)TB * TE;
+  // halo publication slot of this lane (own lanes) and source tile column
+  // of this lane (halo lanes)
+  const int lane_pub = e - e_own0;
+  const int ote = lane_valid ? (e / TE) : 0;
+  const int oe0 = ote * TE;
+  const int ow = min(nE, oe0 + TE) - oe0;
+  const int lane_src = e - oe0;
 
   const int nsteps = nA - 1;
   if (nsteps <= 0) return;
   const int a_first = (dir > 0) ? 1 : nA - 2;
 
-  // row bookkeeping (uniform per wave)
-  int rb[R + 2];  // global B index of rows -1..R relative to the wave
+  // uniform row bookkeeping: rows -1..R relative to this wave
+  int rb[R + 2];
+  bool rvalid[R + 2];
+  long long roff[R + 2];
+  bool rows_all_valid = true;
 #pragma unroll
-  for (int r = 0; r < R + 2; ++r) rb[r] = bw0 + w * R + r - 1;
+  for (int r = 0; r < R + 2; ++r) {
+    rb[r] = bw0 + w * R + r - 1;
+    rvalid[r] = rb[r] >= 0 && rb[r] < nB;
+    rows_all_valid = rows_all_valid && rvalid[r];
+    roff[r] = (long long)min(max(rb[r], 0), nB - 1) * sB;
+  }
+  (void)rows_all_valid;
 
-  auto row_ok = [&](int b) { return b >= 0 && b < nB; };
-
+  // Prefetch loads are raw loads at clamped coordinates (always in bounds) and
+  // carry no masking, so no instruction waits on them until the slab is
+  // consumed.  Values at pad positions (outside the plane) are never used:
+  // a pad's own result is forced to +inf after every step, and its image
+  // value only enters candidates whose distance operand is that +inf.
   auto load_slab = [&](Slab<R, CT> &S, int a) {
-    const long long base = (long long)a * sA;
+    const gfloat *ia = img + (long long)a * sA;
+    const gfloat *da = dist + (long long)a * sA;
 #pragma unroll
     for (int r = 0; r < R + 2; ++r) {
-      const int b = rb[r];
-      const bool ok = row_ok(b) && lane_valid;
-      const long long off = base + (long long)b * sB + e;
 #pragma unroll
-      for (int c = 0; c < CT; ++c) S.I[c][r] = ok ? img[c * cs + off] : 0.0f;
-      if (r >= 1 && r <= R) S.d[r - 1] = ok ? dist[off] : kInf;
+      for (int c = 0; c < CT; ++c) S.I[c][r] = ld_at(ia + (c * cs + roff[r]), ecb);
+      if (r >= 1 && r <= R) S.d[r - 1] = ld_at(da + roff[r], ecb);
     }
   };
 
-  Front<NW, R, CT> F;
-  {
-    const int af = a_first - dir;  // front slice: not modified by this sweep
-    Slab<R, CT> S;
-    load_slab(S, af);
+  // per-row validity of this lane (pad positions hold +inf in the front) and
+  // ownership of each wave row (uniform)
+  bool pvalid[R], row_own[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) F.d[r] = S.d[r];
-#pragma unroll
-    for (int c = 0; c < CT; ++c)
-#pragma unroll
-      for (int r = 0; r < R + 2; ++r) F.I[c][r] = S.I[c][r];
+  for (int r = 0; r < R; ++r) {
+    pvalid[r] = rvalid[r + 1] && lane_valid;
+    row_own[r] = rb[r + 1] >= b_own0 && rb[r + 1] < b_own1;
   }
 
-  Slab<R, CT> S0, S1;
-  load_slab(S0, a_first);
-  if (nsteps > 1) load_slab(S1, a_first + dir);
+  // Slab ring: the slab of step s is S[s % P]; its image rows are also the
+  // front image of step s + 1, so a slab is refilled (with the slice of step
+  // s + P) only after step s + 1 has used it; the front of step 0 starts in
+  // S[P - 1].  Slabs stay in fixed registers this way.
+  float Fd[R];  // front distance rows 0..R-1
+  Slab<R, CT> S[P];
+  load_slab(S[P - 1], a_first - dir);  // front slice: not modified by this sweep
+#pragma unroll
+  for (int r = 0; r < R; ++r) Fd[r] = pvalid[r] ? S[P - 1].d[r] : kInf;
+#pragma unroll
+  for (int p = 0; p < P - 1; ++p) load_slab(S[p], min(max(a_first + p * dir, 0), nA - 1));
 
-  auto step = [&](Slab<R, CT> &S, int s, int slot) {
+  // Publishes this tile's own values of the current front and reloads the
+  // halo positions of the window from the neighbours' publications.  The
+  // prefetch of the next slice is issued between the halo loads and the wait
+  // for them, so that wait does not also wait for the new prefetch.
+  auto exchange = [&](int blk, Slab<R, CT> &pf_slab, int pf_step) {
+    const unsigned tag = tag_base + (unsigned)blk;
+    const long long hslot = (long long)(blk & 1) * tile_slots;
+    gu64 *pub = halo + (long long)lt * 2 * tile_slots + hslot;
+    const int own_w = e_own1 - e_own0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int b = rb[r + 1];
+      if (b >= b_own0 && b < b_own1 && lane_own)
+        __hip_atomic_store(pub + (long long)(b - b_own0) * own_w + lane_pub, pack(Fd[r], tag), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const gu64 *src[R];
+    bool need[R];
+    unsigned long long x[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int b = rb[r + 1];
+      const bool row_own = b >= b_own0 && b < b_own1;
+      need[r] = rvalid[r + 1] && lane_valid && !(row_own && lane_own);
+      const int bq = min(max(b, 0), nB - 1);
+      const int otb = bq / TB;  // uniform
+      src[r] = halo + (long long)(otb * nTE + ote) * 2 * tile_slots + hslot + (long long)(bq - otb * TB) * ow +
+               lane_src;
+      x[r] = 0;
+      if (need[r]) x[r] = __hip_atomic_load(src[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    load_slab(pf_slab, min(max(a_first + pf_step * dir, 0), nA - 1));
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      if (need[r]) {
+        int spins = 0;
+        while ((unsigned)(x[r] >> 32) != tag) {
+          if (++spins > spin_limit) {
+            __hip_atomic_store(status, FG_EWATCHDOG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          x[r] = __hip_atomic_load(src[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        Fd[r] = __uint_as_float((unsigned)x[r]);
+      }
+    }
+  };
+
+  // One slice: Sl holds the new slice, Fr the previous one (front image).
+  // Afterwards Fr is refilled with the slice of step s + P - 1.
+  auto step = [&](Slab<R, CT> &Sl, Slab<R, CT> &Fr, int s, bool prefetch) {
     const int a = a_first + s * dir;
+    const int slot = s & 1;
     // wave-boundary front rows through LDS
+    // (branch-free: the edge waves read a valid slot and select +inf)
     float top = kInf, bot = kInf;
     if constexpr (NW > 1) {
-      edge_first[slot][w][lane] = F.d[0];
-      edge_last[slot][w][lane] = F.d[R - 1];
+      edge_first[slot][w][lane] = Fd[0];
+      edge_last[slot][w][lane] = Fd[R - 1];
       __syncthreads();
-      if (w > 0) top = edge_last[slot][w - 1][lane];
-      if (w < NW - 1) bot = edge_first[slot][w + 1][lane];
+      const float t = edge_last[slot][max(w - 1, 0)][lane];
+      const float u = edge_first[slot][min(w + 1, NW - 1)][lane];
+      top = (w > 0) ? t : kInf;
+      bot = (w < NW - 1) ? u : kInf;
     }
     float nd[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       float dq[3];
-      dq[0] = (r == 0) ? top : F.d[r - 1];
-      dq[1] = F.d[r];
-      dq[2] = (r == R - 1) ? bot : F.d[r + 1];
+      dq[0] = (r == 0) ? top : Fd[r - 1];
+      dq[1] = Fd[r];
+      dq[2] = (r == R - 1) ? bot : Fd[r + 1];
       float Iq[3][CT], In[CT];
 #pragma unroll
       for (int c = 0; c < CT; ++c) {
-        Iq[0][c] = F.I[c][r];
-        Iq[1][c] = F.I[c][r + 1];
-        Iq[2][c] = F.I[c][r + 2];
-        In[c] = S.I[c][r + 1];
+        Iq[0][c] = Fr.I[c][r];
+        Iq[1][c] = Fr.I[c][r + 1];
+        Iq[2][c] = Fr.I[c][r + 2];
+        In[c] = Sl.I[c][r + 1];
       }
-      float v = relax<CT, LAMB1, FAST>(dq, Iq, In, S.d[r], e1, lg);
-      if (pad_lanes) v = lane_valid ? v : kInf;
-      nd[r] = row_ok(rb[r + 1]) ? v : kInf;
+      const float v = relax<CT, LAMB1, FAST>(dq, Iq, In, Sl.d[r], e1, lg);
+      nd[r] = pvalid[r] ? v : kInf;
     }
-    // own outputs
-    const long long base = (long long)a * sA;
+    // own outputs: raw buffer stores, non-own lanes and rows get an
+    // out-of-range offset and the hardware drops their writes (no branch)
+    float *da = Dd.d + (long long)a * sA;
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const int b = rb[r + 1];
-      if (b >= b_own0 && b < b_own1 && lane_own) dist[base + (long long)b * sB + e] = nd[r];
+      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(da + roff[r + 1], 0, 0x7FFFFFF0, 0x00020000);
+      const unsigned off = (row_own[r] && lane_own) ? ecb : 0xFFFFFFFFu;
+      __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(nd[r]), rs, (int)off, 0, 0);
     }
 #pragma unroll
-    for (int r = 0; r < R; ++r) F.d[r] = nd[r];
-#pragma unroll
-    for (int c = 0; c < CT; ++c)
-#pragma unroll
-      for (int r = 0; r < R + 2; ++r) F.I[c][r] = S.I[c][r];
-    // prefetch slice s + 2 into the slab just consumed
-    if (s + 2 < nsteps) load_slab(S, a + 2 * dir);
-    // halo exchange every K slices
-    if (((s + 1) % K) == 0 && (s + 1) < nsteps) {
-      const int blk = (s + 1) / K;
-      const unsigned tag = tag_base + (unsigned)blk;
-      const int hslot = blk & 1;
-      unsigned long long *pub = my_halo + (long long)hslot * TB * TE;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int b = rb[r + 1];
-        if (b >= b_own0 && b < b_own1 && lane_own)
-          __hip_atomic_store(pub + (long long)(b - b_own0) * own_w + (e - e_own0), pack(F.d[r], tag),
-                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        const int b = rb[r + 1];
-        const bool inside = row_ok(b) && lane_valid;
-        const bool own = b >= b_own0 && b < b_own1 && lane_own;
-        if (inside && !own) {
-          const int otb = b / TB, ote = e / TE;
-          const int ob0 = otb * TB, oe0 = ote * TE;
-          const int ow = min(nE, oe0 + TE) - oe0;
-          const unsigned long long *src = halo_ro + Dd.halo0 +
-                                          (long long)(otb * nTE + ote) * 2 * TB * TE +
-                                          (long long)hslot * TB * TE + (long long)(b - ob0) * ow +
-                                          (e - oe0);
-          unsigned long long x;
-          int spins = 0;
-          while (true) {
-            x = __hip_atomic_load(src, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if ((unsigned)(x >> 32) == tag) break;
-            if (++spins > spin_limit) {
-              __hip_atomic_store(status, FG_EWATCHDOG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              break;
-            }
-          }
-          F.d[r] = __uint_as_float((unsigned)x);
-        }
-      }
-    }
+    for (int r = 0; r < R; ++r) Fd[r] = nd[r];
+    // prefetch without a branch: near the end of the sweep the slice index is
+    // clamped and the (valid, unused) data is simply never consumed
+    if (prefetch) load_slab(Fr, min(max(a + (P - 1) * dir, 0), nA - 1));
   };
 
+  // Blocks of K steps (K is a multiple of P, so every block starts at step
+  // s with s % P == 0); halo exchange between blocks.  No branch separates
+  // the steps of a group of P, which keeps the waits on prefetched slabs
+  // precise.
   int s = 0;
-  for (; s + 1 < nsteps; s += 2) {
-    step(S0, s, 0);
-    step(S1, s + 1, 1);
+  while (true) {
+    const int blk_end = min(s + K, nsteps);
+    if (blk_end == nsteps) {
+      for (; s + P <= nsteps; s += P) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) step(S[p], S[(p + P - 1) % P], s + p, true);
+      }
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+        if (s + p < nsteps) step(S[p], S[(p + P - 1) % P], s + p, true);
+      break;
+    }
+    for (; s + P < blk_end; s += P) {
+#pragma unroll
+      for (int p = 0; p < P; ++p) step(S[p], S[(p + P - 1) % P], s + p, true);
+    }
+#pragma unroll
+    for (int p = 0; p < P - 1; ++p) step(S[p], S[(p + P - 1) % P], s + p, true);
+    step(S[P - 1], S[P - 2], s + P - 1, false);
+    s += P;
+    // the deferred prefetch of step s - 1 goes into S[P - 2]: slice of step s - 1 + P - 1
+    exchange(s / K, S[P - 2], s + P - 2);
   }
-  if (s < nsteps) step(S0, s, 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -536,7 +613,7 @@ __global__ void fill_i32_kernel(int *p, int v) { *p = v; }
 // ---------------------------------------------------------------------------
 
 thread_local std::string g_err;
-thread_local fg_tuning g_tune = {1, 4, 0};
+thread_local fg_tuning g_tune = {1, 4, 0, 0};
 
 // Per-launch timing of tiled sweeps (fg_profile_enable / fg_profile_collect).
 struct ProfRec {
@@ -596,21 +673,30 @@ struct Xform {
 // Tiled plan for one sweep plane.
 struct Plan {
   int NW, R;
+  int K;  // halo depth actually used: the requested depth rounded up to a multiple of P
   int nTB, nTE, TB, TE;
   bool ok;
 };
 
 struct Shape {
   int NW, R;
+  int P12, P34;  // slab ring size for 1-2 / 3-4 image channels (prefetch distance P - 1)
 };
-constexpr Shape kShapes[] = {{1, 1}, {4, 8}};
+// Workgroup shapes with compiled kernels: index 0 serves 2D row sweeps
+// (one row); the others tile 3D planes (selected by fg_tuning.shape).
+constexpr Shape kShapes[] = {{1, 1, 4, 4}, {4, 8, 2, 2}, {8, 4, 3, 2}, {16, 2, 3, 3}};
+constexpr int kNumShapes = 4;
+thread_local int g_shape = 2;
 
-Plan plan_plane(int nB, int nE, int K) {
+Plan plan_plane(int nB, int nE, int Kreq, int C) {
   Plan p{};
   p.ok = false;
-  const Shape s = (nB == 1) ? kShapes[0] : kShapes[1];
+  const Shape s = (nB == 1) ? kShapes[0] : kShapes[g_shape];
   p.NW = s.NW;
   p.R = s.R;
+  const int P = C <= 2 ? s.P12 : s.P34;
+  const int K = ((Kreq + P - 1) / P) * P;
+  p.K = K;
   const int WR = s.NW * s.R;
   if (nE <= 62) {
     p.nTE = 1;
@@ -648,17 +734,19 @@ void pass_plane(const Xform &x, int pass, int &nA, int &nB, int &nE) {
   else { nA = x.W; nB = x.H; nE = x.D; }
 }
 
-template <int NW, int R, int CT, bool L1, bool FA>
+template <int NW, int R, int P, int CT, bool L1, bool FA>
 const void *kernel_ptr() {
-  return reinterpret_cast<const void *>(&sweep_tiled_kernel<NW, R, CT, L1, FA>);
+  return reinterpret_cast<const void *>(&sweep_tiled_kernel<NW, R, P, CT, L1, FA>);
 }
 
-template <int NW, int R>
+// lamb1 implies fast (l_grad == 1 and l_eucl*LD == +0 are non-negative)
+template <int NW, int R, int P>
 const void *pick_kernel_nr(int C, bool lamb1, bool fast) {
-#define FG_K(CT)                                                                               \
-  if (C == CT) {                                                                               \
-    if (fast) return lamb1 ? kernel_ptr<NW, R, CT, true, true>() : kernel_ptr<NW, R, CT, false, true>(); \
-    return lamb1 ? kernel_ptr<NW, R, CT, true, false>() : kernel_ptr<NW, R, CT, false, false>(); \
+#define FG_K(CT)                                                             \
+  if (C == CT) {                                                             \
+    if (lamb1) return kernel_ptr<NW, R, P, CT, true, true>();                \
+    return fast ? kernel_ptr<NW, R, P, CT, false, true>()                    \
+                : kernel_ptr<NW, R, P, CT, false, false>();                  \
   }
   FG_K(1)
   FG_K(2)
@@ -669,8 +757,18 @@ const void *pick_kernel_nr(int C, bool lamb1, bool fast) {
 }
 
 const void *pick_kernel(int NW, int R, int C, bool lamb1, bool fast) {
-  if (NW == 1 && R == 1) return pick_kernel_nr<1, 1>(C, lamb1, fast);
-  if (NW == 4 && R == 8) return pick_kernel_nr<4, 8>(C, lamb1, fast);
+  for (const Shape &s : kShapes) {
+    if (s.NW != NW || s.R != R) continue;
+    const int P = C <= 2 ? s.P12 : s.P34;
+#define FG_S(nw, r, p) \
+    if (NW == nw && R == r && P == p) return pick_kernel_nr<nw, r, p>(C, lamb1, fast);
+    FG_S(1, 1, 4)
+    FG_S(4, 8, 2)
+    FG_S(8, 4, 3)
+    FG_S(8, 4, 2)
+    FG_S(16, 2, 3)
+#undef FG_S
+  }
   return nullptr;
 }
 
@@ -810,20 +908,21 @@ struct Runner {
   int record_tiled(std::vector<SweepJob> &jobs, int dir, int NW, int R, int C, bool lamb1, bool fast) {
     std::vector<SweepDesc> descs;
     long long halo_used = 0;
-    int tiles = 0, max_blocks = 0;
+    int tiles = 0, max_blocks = 0, kk = K;
     double alg = 0;
     for (auto &j : jobs) {
       SweepDesc s;
       fill_desc(*j.x, j.pass, dir, s);
       alg += (double)(s.nA - 1) * s.nB * s.nE * (4.0 * s.C + 8.0);
-      Plan p = plan_plane(s.nB, s.nE, K);
+      Plan p = plan_plane(s.nB, s.nE, K, s.C);
       s.nTB = p.nTB; s.nTE = p.nTE; s.TB = p.TB; s.TE = p.TE;
       s.tile0 = tiles;
       s.halo0 = halo_used;
       const int nt = p.nTB * p.nTE;
       tiles += nt;
       halo_used += (long long)nt * 2 * p.TB * p.TE;
-      max_blocks = std::max(max_blocks, (s.nA - 1) / K + 1);
+      kk = p.K;
+      max_blocks = std::max(max_blocks, (s.nA - 1) / p.K + 1);
       descs.push_back(s);
     }
     if (descs.empty()) return FG_OK;
@@ -839,7 +938,7 @@ struct Runner {
     c.block = NW * 64;
     c.off = rec.put(descs.data(), descs.size() * sizeof(SweepDesc));
     c.ndesc = (int)descs.size();
-    c.K = K;
+    c.K = kk;
     c.tag_base = tag_base;
     c.alg_bytes = alg;
     rec.cmds.push_back(c);
@@ -911,7 +1010,7 @@ struct Runner {
         int nA, nB, nE;
         pass_plane(*x, pass, nA, nB, nE);
         if (nA <= 1) continue;
-        Plan p = plan_plane(nB, nE, K);
+        Plan p = plan_plane(nB, nE, K, x->C);
         if (engine == 0 || !p.ok || x->C > 4) {
           record_per_slice(*x, pass, dir);
           continue;
@@ -946,7 +1045,7 @@ struct Runner {
     for (auto &j : jobs) {
       int nA, nB, nE;
       pass_plane(*j.x, j.pass, nA, nB, nE);
-      Plan p = plan_plane(nB, nE, K);
+      Plan p = plan_plane(nB, nE, K, j.x->C);
       const long long nt = (long long)p.nTB * p.nTE;
       if (nt > cap) {  // one plane alone cannot be co-resident: per-slice engine
         record_per_slice(*j.x, j.pass, dir);
@@ -1104,7 +1203,7 @@ size_t halo_entries_for(const std::vector<OpSpec> &ops, int K) {
     for (int p = 0; p < 3; ++p) {
       int nA, nB, nE;
       pass_plane(x, p, nA, nB, nE);
-      Plan pl = plan_plane(nB, nE, K);
+      Plan pl = plan_plane(nB, nE, K, o.C);
       if (!pl.ok) continue;
       worst = std::max(worst, (size_t)pl.nTB * pl.nTE * 2 * pl.TB * pl.TE);
     }
@@ -1360,7 +1459,7 @@ size_t fg_workspace_bytes(int op, int C, int D, int H, int W) {
   OpSpec o{};
   o.op = (op < FG_OP_UNSIGNED || op > FG_OP_GSF) ? FG_OP_GSF : op;
   o.C = std::max(C, 1); o.Cm = 1; o.D = std::max(D, 1); o.H = std::max(H, 1); o.W = std::max(W, 1);
-  o.is2d = (D <= 1);
+  o.is2d = (D <= 0);
   o.iters = 1;
   std::vector<OpSpec> ops{o};
   return workspace_for(ops, current_halo());
@@ -1449,7 +1548,9 @@ int fg_tuning_set(const fg_tuning *t) {
   if (t->engine < 0 || t->engine > 1) return fail(FG_EINVAL, "engine must be 0 or 1");
   if (t->halo < 1 || t->halo > kMaxK) return fail(FG_EINVAL, "halo must be in [1, 16]");
   if (t->max_groups < 0) return fail(FG_EINVAL, "max_groups must be >= 0");
+  if (t->shape < 0 || t->shape >= kNumShapes) return fail(FG_EINVAL, "shape must be in [0, 3]");
   g_tune = *t;
+  g_shape = t->shape == 0 ? 2 : t->shape;
   return FG_OK;
 }
 

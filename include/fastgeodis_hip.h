@@ -53,7 +53,7 @@ int fg_abi_version(void);
 const char *fg_last_error(void);
 
 /* Bytes of device workspace one call of kind `op` needs for an image of C
- * channels over D*H*W voxels (D <= 1 selects 2D).  Depends on the tuning
+ * channels over D*H*W voxels; D <= 0 selects the 2D entry points (H*W).  Depends on the tuning
  * (fg_tuning_set) of the calling host thread. */
 size_t fg_workspace_bytes(int op, int C, int D, int H, int W);
 
@@ -136,6 +136,7 @@ typedef struct fg_tuning {
   int engine;
   int halo;
   int max_groups; /* cap on persistent workgroups (0 = occupancy limit) */
+  int shape;      /* 3D plane workgroup shape: 0 auto, 1 = 4x8, 2 = 8x4, 3 = 16x2 (waves x rows) */
 } fg_tuning;
 int fg_tuning_set(const fg_tuning *t);
 int fg_tuning_get(fg_tuning *t);

@@ -57,7 +57,7 @@ def test_workspace_query(lib):
     q = lib.fg_workspace_bytes
     q.restype = ctypes.c_size_t
     q.argtypes = [ctypes.c_int] * 5
-    small = q(0, 1, 1, 64, 64)
+    small = q(0, 1, 0, 64, 64)
     big = q(0, 1, 64, 64, 64)
     gsf = q(2, 3, 64, 64, 64)
     assert 0 < small < big < gsf
@@ -68,8 +68,10 @@ def test_workspace_query(lib):
 def test_tuning_roundtrip():
     old = FG.get_tuning()
     try:
-        FG.set_tuning(0, 3, 17)
-        assert FG.get_tuning() == {"engine": 0, "halo": 3, "max_groups": 17}
+        FG.set_tuning(0, 3, 17, 3)
+        assert FG.get_tuning() == {"engine": 0, "halo": 3, "max_groups": 17, "shape": 3}
+        with pytest.raises(ValueError):
+            FG.set_tuning(1, 4, 0, 4)
         with pytest.raises(ValueError):
             FG.set_tuning(2, 4, 0)
         with pytest.raises(ValueError):

@@ -44,11 +44,15 @@ def diff_report(a, b):
     return f"{bad.sum()} mismatches; first {idx[:3].tolist()} got {a[bad][:3]} want {b[bad][:3]}"
 
 
-@pytest.fixture(params=[(1, 4), (1, 1), (1, 3), (1, 7), (0, 4)], ids=["tiled-K4", "tiled-K1", "tiled-K3", "tiled-K7", "per-slice"])
+ENGINES = [(1, 4, 0), (1, 1, 1), (1, 3, 3), (1, 7, 2), (1, 2, 1), (1, 5, 3), (0, 4, 0)]
+
+
+@pytest.fixture(params=ENGINES, ids=["tiled-K4-auto", "tiled-K1-4x8", "tiled-K3-16x2", "tiled-K7-8x4",
+                                     "tiled-K2-4x8", "tiled-K5-16x2", "per-slice"])
 def engine(request, device):
-    e, k = request.param
+    e, k, shape = request.param
     old = FG.get_tuning()
-    FG.set_tuning(e, k, 0)
+    FG.set_tuning(e, k, 0, shape)
     yield request.param
     FG.set_tuning(**old)
 
