@@ -10,7 +10,8 @@ order, defaults and exception types::
 
 Computation happens in hand-written HIP kernels (``csrc/fg_kernels.hip``)
 behind a C ABI (``include/fastgeodis_hip.h``), reached through the torch
-extension ``FastGeodisCpp`` (``csrc/fg_glue.cpp``).  There is no CPU
+extension ``_fastgeodis_hip`` (``csrc/fg_glue.cpp``), exposed as
+``fastgeodis_amd.FastGeodisCpp`` like the reference's binding module.  There is no CPU
 implementation: HIP tensors are processed on their device and the current
 torch stream; CPU tensors are staged through the current HIP device.  If the
 native modules are not built, importing this package raises ImportError.
@@ -36,18 +37,18 @@ _LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
 def _load_native():
     import sysconfig
 
-    path = os.path.join(_LIB_DIR, "FastGeodisCpp" + sysconfig.get_config_var("EXT_SUFFIX"))
+    path = os.path.join(_LIB_DIR, "_fastgeodis_hip" + sysconfig.get_config_var("EXT_SUFFIX"))
     if not os.path.isfile(path):
         raise ImportError(
             f"fastgeodis_amd native module not built ({path} missing); run "
-            "`python -m fastgeodis_amd._build` (hipcc for gfx950)")
-    spec = importlib.util.spec_from_file_location("FastGeodisCpp", path)
+            "`python fastgeodis_amd/_build.py` (hipcc for gfx950)")
+    spec = importlib.util.spec_from_file_location("_fastgeodis_hip", path)
     mod = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(mod)
-    sys.modules.setdefault("FastGeodisCpp", mod)
     return mod
 
 
+# The binding module; the reference's wrappers call theirs FastGeodisCpp.
 FastGeodisCpp = _load_native()
 
 
@@ -179,7 +180,8 @@ def geodesic_batch(instances: Sequence[Instance]) -> List[torch.Tensor]:
 def set_tuning(engine: int = 1, halo: int = 4, max_groups: int = 0, shape: int = 0) -> None:
     """Engine (0 per-slice, 1 persistent tiled), halo depth K, workgroup cap
     (0 = occupancy limit) and 3D plane workgroup shape (0 auto, 1 = 4 waves x
-    8 rows, 2 = 8 x 4, 3 = 16 x 2) used by later calls on this host thread."""
+    8 rows, 2 = 8 x 4, 3 = 16 x 2, 4 = 8 x 4 with a deeper prefetch ring) used
+    by later calls on this host thread."""
     FastGeodisCpp.set_tuning(int(engine), int(halo), int(max_groups), int(shape))
 
 
@@ -193,7 +195,7 @@ def native_library_paths() -> List[str]:
     import sysconfig
 
     return [os.path.join(_LIB_DIR, "libfastgeodis_hip.so"),
-            os.path.join(_LIB_DIR, "FastGeodisCpp" + sysconfig.get_config_var("EXT_SUFFIX"))]
+            os.path.join(_LIB_DIR, "_fastgeodis_hip" + sysconfig.get_config_var("EXT_SUFFIX"))]
 
 
 __all__ = [

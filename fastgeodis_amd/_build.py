@@ -4,11 +4,15 @@
   gfx950 with ``-ffp-contract=off`` (no FMA, bit-exact two-rounding relax),
   without fast-math, and with packed FP32 instructions disabled.  The device
   assembly is checked for scalar-cache stores, which this library never uses.
-* ``FastGeodisCpp*.so``: ``csrc/fg_glue.cpp``, the torch extension that mirrors
-  the reference's ``FastGeodisCpp`` module (validation, dispatch, workspace
-  from the caching allocator) and calls the C ABI above.
+* ``_fastgeodis_hip*.so``: ``csrc/fg_glue.cpp``, the torch extension that
+  mirrors the reference's ``FastGeodisCpp`` module (validation, dispatch,
+  workspace from the caching allocator) and calls the C ABI above.  It has
+  its own module name so it can be loaded beside the reference's module.
 
-Run ``python -m fastgeodis_amd._build`` or ``__graft_entry__.build()``.
+Run ``python fastgeodis_amd/_build.py [hip|glue|all]`` or
+``__graft_entry__.build()``.  The recipe is loaded by path, not through the
+package, because the package refuses to import while its native module is
+missing.
 """
 from __future__ import annotations
 
@@ -43,9 +47,18 @@ def _run(cmd, **kw):
     return r
 
 
+def _asm_guard():
+    import importlib.util
+
+    spec = importlib.util.spec_from_file_location("fg_asm_guard", os.path.join(HERE, "_asm_guard.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
 def check_device_asm(src: str) -> None:
     """Compile the device side to assembly and refuse scalar-cache writes."""
-    from ._asm_guard import scalar_cache_writes
+    scalar_cache_writes = _asm_guard().scalar_cache_writes
 
     asm = os.path.join(LIB_DIR, "fg_kernels.s")
     _run([HIPCC, *HIP_FLAGS, "-I", INCLUDE, "--cuda-device-only", "-S", src, "-o", asm])
@@ -76,14 +89,14 @@ def build_glue(verbose: bool = False) -> str:
 
     os.makedirs(LIB_DIR, exist_ok=True)
     src = os.path.join(CSRC, "fg_glue.cpp")
-    out = os.path.join(LIB_DIR, "FastGeodisCpp" + sysconfig.get_config_var("EXT_SUFFIX"))
+    out = os.path.join(LIB_DIR, "_fastgeodis_hip" + sysconfig.get_config_var("EXT_SUFFIX"))
     abi = int(torch._C._GLIBCXX_USE_CXX11_ABI)
     inc = ["-I" + INCLUDE, "-I/opt/rocm/include", "-I" + sysconfig.get_paths()["include"]]
     inc += ["-I" + p for p in ce.include_paths()]
     cmd = [
-        "g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-g0", "-w",
+        "g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-g0", "-w", "-fvisibility=hidden",
         "-D__HIP_PLATFORM_AMD__=1", "-DUSE_ROCM=1",
-        "-DTORCH_API_INCLUDE_EXTENSION_H", "-DTORCH_EXTENSION_NAME=FastGeodisCpp",
+        "-DTORCH_API_INCLUDE_EXTENSION_H", "-DTORCH_EXTENSION_NAME=_fastgeodis_hip",
         f"-D_GLIBCXX_USE_CXX11_ABI={abi}", *inc, src, "-o", out + ".tmp",
         "-L" + LIB_DIR, "-lfastgeodis_hip", "-Wl,-rpath,$ORIGIN",
     ]

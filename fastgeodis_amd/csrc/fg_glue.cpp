@@ -1,4 +1,4 @@
-// fg_glue.cpp -- torch extension "FastGeodisCpp" over the gfx950 C ABI.
+// fg_glue.cpp -- torch extension "_fastgeodis_hip" over the gfx950 C ABI.
 //
 // Mirrors the reference's binding module (FastGeodis/fastgeodis.cpp:44-146,
 // 241-258, 319-343, 423-451) for the raster-scan entry points: the same
@@ -172,8 +172,6 @@ torch::Tensor dispatch(const Call &c, const torch::Tensor &image, const torch::T
   return out.to(torch::kCPU);
 }
 
-}  // namespace
-
 // ---- reference-compatible entry points (fastgeodis.cpp) -------------------
 
 torch::Tensor generalised_geodesic2d(const torch::Tensor &image, const torch::Tensor &mask, const float &v,
@@ -313,6 +311,8 @@ std::tuple<int64_t, double, double> profile_collect() {
 int64_t workspace_bytes(int64_t op, int64_t C, int64_t D, int64_t H, int64_t W) {
   return (int64_t)fg_workspace_bytes((int)op, (int)C, (int)D, (int)H, (int)W);
 }
+
+}  // namespace
 
 PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("generalised_geodesic2d", &generalised_geodesic2d, "Generalised Geodesic distance 2d");

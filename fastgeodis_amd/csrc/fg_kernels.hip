@@ -684,8 +684,8 @@ struct Shape {
 };
 // Workgroup shapes with compiled kernels: index 0 serves 2D row sweeps
 // (one row); the others tile 3D planes (selected by fg_tuning.shape).
-constexpr Shape kShapes[] = {{1, 1, 4, 4}, {4, 8, 2, 2}, {8, 4, 3, 2}, {16, 2, 3, 3}};
-constexpr int kNumShapes = 4;
+constexpr Shape kShapes[] = {{1, 1, 4, 4}, {4, 8, 2, 2}, {8, 4, 3, 2}, {16, 2, 3, 3}, {8, 4, 5, 2}};
+constexpr int kNumShapes = 5;
 thread_local int g_shape = 2;
 
 Plan plan_plane(int nB, int nE, int Kreq, int C) {
@@ -739,20 +739,24 @@ const void *kernel_ptr() {
   return reinterpret_cast<const void *>(&sweep_tiled_kernel<NW, R, P, CT, L1, FA>);
 }
 
-// lamb1 implies fast (l_grad == 1 and l_eucl*LD == +0 are non-negative)
-template <int NW, int R, int P>
+// lamb1 implies fast (l_grad == 1 and l_eucl*LD == +0 are non-negative).
+// Only the channel counts a (shape, ring) pair serves are instantiated.
+template <int NW, int R, int P, int CT>
+const void *pick_variant(bool lamb1, bool fast) {
+  if (lamb1) return kernel_ptr<NW, R, P, CT, true, true>();
+  return fast ? kernel_ptr<NW, R, P, CT, false, true>() : kernel_ptr<NW, R, P, CT, false, false>();
+}
+
+template <int NW, int R, int P, int CLO, int CHI>
 const void *pick_kernel_nr(int C, bool lamb1, bool fast) {
-#define FG_K(CT)                                                             \
-  if (C == CT) {                                                             \
-    if (lamb1) return kernel_ptr<NW, R, P, CT, true, true>();                \
-    return fast ? kernel_ptr<NW, R, P, CT, false, true>()                    \
-                : kernel_ptr<NW, R, P, CT, false, false>();                  \
-  }
-  FG_K(1)
-  FG_K(2)
-  FG_K(3)
-  FG_K(4)
-#undef FG_K
+  if constexpr (CLO <= 1 && 1 <= CHI)
+    if (C == 1) return pick_variant<NW, R, P, 1>(lamb1, fast);
+  if constexpr (CLO <= 2 && 2 <= CHI)
+    if (C == 2) return pick_variant<NW, R, P, 2>(lamb1, fast);
+  if constexpr (CLO <= 3 && 3 <= CHI)
+    if (C == 3) return pick_variant<NW, R, P, 3>(lamb1, fast);
+  if constexpr (CLO <= 4 && 4 <= CHI)
+    if (C == 4) return pick_variant<NW, R, P, 4>(lamb1, fast);
   return nullptr;
 }
 
@@ -760,13 +764,14 @@ const void *pick_kernel(int NW, int R, int C, bool lamb1, bool fast) {
   for (const Shape &s : kShapes) {
     if (s.NW != NW || s.R != R) continue;
     const int P = C <= 2 ? s.P12 : s.P34;
-#define FG_S(nw, r, p) \
-    if (NW == nw && R == r && P == p) return pick_kernel_nr<nw, r, p>(C, lamb1, fast);
-    FG_S(1, 1, 4)
-    FG_S(4, 8, 2)
-    FG_S(8, 4, 3)
-    FG_S(8, 4, 2)
-    FG_S(16, 2, 3)
+#define FG_S(nw, r, p, clo, chi) \
+    if (NW == nw && R == r && P == p && C >= clo && C <= chi) return pick_kernel_nr<nw, r, p, clo, chi>(C, lamb1, fast);
+    FG_S(1, 1, 4, 1, 4)
+    FG_S(4, 8, 2, 1, 4)
+    FG_S(8, 4, 3, 1, 2)
+    FG_S(8, 4, 5, 1, 2)
+    FG_S(8, 4, 2, 3, 4)
+    FG_S(16, 2, 3, 1, 4)
 #undef FG_S
   }
   return nullptr;
@@ -1548,7 +1553,7 @@ int fg_tuning_set(const fg_tuning *t) {
   if (t->engine < 0 || t->engine > 1) return fail(FG_EINVAL, "engine must be 0 or 1");
   if (t->halo < 1 || t->halo > kMaxK) return fail(FG_EINVAL, "halo must be in [1, 16]");
   if (t->max_groups < 0) return fail(FG_EINVAL, "max_groups must be >= 0");
-  if (t->shape < 0 || t->shape >= kNumShapes) return fail(FG_EINVAL, "shape must be in [0, 3]");
+  if (t->shape < 0 || t->shape >= kNumShapes) return fail(FG_EINVAL, "shape must be in [0, 4]");
   g_tune = *t;
   g_shape = t->shape == 0 ? 2 : t->shape;
   return FG_OK;

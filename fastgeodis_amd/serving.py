@@ -1,0 +1,282 @@
+"""Interactive-segmentation serving loop.
+
+Volumes stay resident on the device as sessions.  A click adds 1-5 seed
+voxels to a session's soft mask (foreground -> 1, background -> 0; the mask
+starts at 0.5, which makes ``v * 0.5`` effectively infinite) and asks for
+``signed_generalised_geodesic3d(image, mask, spacing, v, lamb, iter)``.
+
+Continuous batching: a scheduler thread keeps a queue of requests.  Whenever
+the device can take more work it admits queued requests, in arrival order,
+while their estimated device memory fits the HBM budget and at most one
+request per session is in the batch (a later click on the same session
+waits for the next batch, because it must see the earlier click's edit).
+Admitted requests run together through the batched entry point, so their
+sweeps share the same persistent launches.  Requests that arrive while a
+batch is on the GPU are admitted into the next batch as soon as the current
+one completes; two batches are kept in flight so the device never waits for
+the host.  Every request's result equals the same request served alone, bit
+for bit (the batched engine computes each instance exactly as the
+single-instance call does).
+
+Latency is measured from a request's arrival to the completion of the HIP
+event recorded after its batch.
+"""
+from __future__ import annotations
+
+import collections
+import math
+import threading
+import time
+from concurrent.futures import Future
+from dataclasses import dataclass, field
+from typing import Dict, List, Optional, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from . import FastGeodisCpp, Instance, geodesic_batch
+
+OP_SIGNED = 1
+
+
+@dataclass
+class Session:
+    image: torch.Tensor            # (1, C, D, H, W) float32, on device
+    mask: torch.Tensor             # (1, 1, D, H, W) float32, on device
+    spacing: Tuple[float, float, float]
+
+    @property
+    def voxels(self) -> int:
+        return int(self.mask.numel())
+
+
+@dataclass
+class Click:
+    session: int
+    voxels: Sequence[Tuple[int, int, int]]
+    foreground: bool
+    arrival: float = field(default_factory=time.perf_counter)
+
+
+@dataclass
+class Result:
+    distance: torch.Tensor         # (1, 1, D, H, W) signed distance, on device
+    latency_s: float
+    batch_size: int
+
+
+class Server:
+    """Continuous-batching server for click requests on resident sessions.
+
+    budget_bytes: device memory the in-flight requests may use (outputs,
+    mask snapshots and the batched call's workspace, estimated per request).
+    """
+
+    def __init__(self, budget_bytes: int = 8 << 30, max_batch: int = 64, v: float = 1e10,
+                 lamb: float = 1.0, iterations: int = 2, device: str = "cuda", inflight: int = 2):
+        self.budget = int(budget_bytes)
+        self.max_batch = int(max_batch)
+        self.v, self.lamb, self.iterations = float(v), float(lamb), int(iterations)
+        self.device = torch.device(device)
+        self.inflight_limit = max(1, int(inflight))
+        self.sessions: Dict[int, Session] = {}
+        self._queue: collections.deque = collections.deque()
+        self._cv = threading.Condition()
+        self._stop = False
+        self._thread: Optional[threading.Thread] = None
+        self._stream = torch.cuda.Stream(device=self.device)
+        self.batches: List[int] = []
+        self.rejected = 0
+
+    # ---- sessions -------------------------------------------------------
+    def add_session(self, sid: int, image: torch.Tensor, spacing=(1.0, 1.0, 1.0)) -> None:
+        if image.dim() != 5 or image.shape[0] != 1:
+            raise ValueError("session image must be (1, C, D, H, W)")
+        image = image.to(self.device, torch.float32).contiguous()
+        mask = torch.full((1, 1) + tuple(image.shape[2:]), 0.5, device=self.device)
+        self.sessions[sid] = Session(image, mask, tuple(float(s) for s in spacing))
+
+    def request_bytes(self, sid: int) -> int:
+        """Device bytes one request of this session holds while in flight:
+        output + mask snapshot + its share of the batched workspace
+        (transposed image, second distance, two transposed distances)."""
+        s = self.sessions[sid]
+        n = s.voxels
+        C = int(s.image.shape[1])
+        return 4 * n * (2 + C + 3) + (1 << 20)
+
+    # ---- requests -------------------------------------------------------
+    def submit(self, click: Click) -> Future:
+        if click.session not in self.sessions:
+            raise KeyError(f"unknown session {click.session}")
+        if self.request_bytes(click.session) > self.budget:
+            raise ValueError("request larger than the HBM budget")
+        fut: Future = Future()
+        with self._cv:
+            self._queue.append((click, fut))
+            self._cv.notify()
+        return fut
+
+    def start(self) -> "Server":
+        self._stop = False
+        self._thread = threading.Thread(target=self._run, name="fastgeodis-serving", daemon=True)
+        self._thread.start()
+        return self
+
+    def stop(self) -> None:
+        with self._cv:
+            self._stop = True
+            self._cv.notify()
+        if self._thread is not None:
+            self._thread.join()
+            self._thread = None
+
+    # ---- scheduler ------------------------------------------------------
+    def _admit(self) -> List[Tuple[Click, Future]]:
+        """Takes queued requests in arrival order while they fit the budget,
+        at most one per session."""
+        batch, used, seen, keep = [], 0, set(), collections.deque()
+        while self._queue and len(batch) < self.max_batch:
+            click, fut = self._queue.popleft()
+            need = self.request_bytes(click.session)
+            if click.session in seen or used + need > self.budget:
+                keep.append((click, fut))
+                if used + need > self.budget:
+                    break
+                continue
+            seen.add(click.session)
+            used += need
+            batch.append((click, fut))
+        # requests left behind keep their order in front of later arrivals
+        while keep:
+            self._queue.appendleft(keep.pop())
+        return batch
+
+    def _launch(self, batch):
+        insts, snaps = [], []
+        with torch.cuda.stream(self._stream):
+            for click, _ in batch:
+                s = self.sessions[click.session]
+                val = 1.0 if click.foreground else 0.0
+                for (z, y, x) in click.voxels:
+                    s.mask[0, 0, z, y, x] = val
+                snap = s.mask.clone()  # this request's view of the mask
+                snaps.append(snap)
+                insts.append(Instance(s.image, snap, self.v, self.lamb, iter=self.iterations,
+                                      spacing=list(s.spacing), op="signed"))
+            outs = geodesic_batch(insts)
+            ev = torch.cuda.Event()
+            ev.record(self._stream)
+        return outs, ev, snaps
+
+    def _run(self):
+        pending = collections.deque()  # (batch, outs, event, snaps)
+        torch.cuda.set_device(self.device)
+        while True:
+            with self._cv:
+                while not self._queue and not pending and not self._stop:
+                    self._cv.wait(timeout=0.05)
+                if self._stop and not self._queue and not pending:
+                    return
+                batch = self._admit() if len(pending) < self.inflight_limit else []
+            if batch:
+                outs, ev, snaps = self._launch(batch)
+                pending.append((batch, outs, ev, snaps))
+                self.batches.append(len(batch))
+                continue
+            if pending:
+                b, outs, ev, snaps = pending[0]
+                if not ev.query():
+                    # let new arrivals be admitted while the device is busy
+                    time.sleep(20e-6)
+                    if len(pending) < self.inflight_limit:
+                        continue
+                    ev.synchronize()
+                pending.popleft()
+                t = time.perf_counter()
+                for (click, fut), out in zip(b, outs):
+                    fut.set_result(Result(out, t - click.arrival, len(b)))
+
+
+def run_alone(server: Server, click: Click, mask_before: torch.Tensor) -> torch.Tensor:
+    """The same request served alone (for checks): applies the click to a copy
+    of the session's mask as it was before and runs the single-instance call."""
+    from . import signed_generalised_geodesic3d
+
+    s = server.sessions[click.session]
+    m = mask_before.clone()
+    val = 1.0 if click.foreground else 0.0
+    for (z, y, x) in click.voxels:
+        m[0, 0, z, y, x] = val
+    return signed_generalised_geodesic3d(s.image, m, list(s.spacing), server.v, server.lamb, server.iterations)
+
+
+def poisson_load(server: Server, qps: float, nreq: int, seed: int = 5,
+                 sessions: Optional[Sequence[int]] = None):
+    """Submits `nreq` clicks with exponential inter-arrival times at `qps` and
+    returns (latencies_s, wall_s, batch sizes)."""
+    rng = np.random.default_rng(seed)
+    sids = list(sessions if sessions is not None else server.sessions.keys())
+    futs = []
+    t0 = time.perf_counter()
+    next_t = t0
+    for _ in range(nreq):
+        next_t += rng.exponential(1.0 / qps)
+        while True:
+            now = time.perf_counter()
+            if now >= next_t:
+                break
+            time.sleep(min(next_t - now, 0.0005))
+        sid = int(rng.choice(sids))
+        shape = server.sessions[sid].mask.shape[2:]
+        k = int(rng.integers(1, 6))
+        vox = [tuple(int(rng.integers(0, s)) for s in shape) for _ in range(k)]
+        futs.append(server.submit(Click(sid, vox, bool(rng.integers(0, 2)))))
+    res = [f.result() for f in futs]
+    wall = time.perf_counter() - t0
+    return [r.latency_s for r in res], wall, [r.batch_size for r in res]
+
+
+def make_sessions(server: Server, n: int, lo: int = 96, hi: int = 320, seed: int = 5,
+                  spacing=(2.0, 1.0, 1.0)) -> None:
+    """n sessions with sides uniform in [lo, hi] and blob-field images."""
+    from .synthetic import blob_field
+
+    rng = np.random.default_rng(seed)
+    for sid in range(n):
+        shape = tuple(int(x) for x in rng.integers(lo, hi + 1, size=3))
+        img = blob_field(shape, 8, (6.0, 30.0), seed * 1000 + sid, server.device)[None, None]
+        server.add_session(sid, img, spacing)
+
+
+def main(argv=None):
+    import argparse
+    import json
+
+    ap = argparse.ArgumentParser(description="click-serving load test (Poisson arrivals)")
+    ap.add_argument("--sessions", type=int, default=64)
+    ap.add_argument("--lo", type=int, default=96)
+    ap.add_argument("--hi", type=int, default=320)
+    ap.add_argument("--qps", type=float, nargs="+", default=[50, 200, 1000])
+    ap.add_argument("--requests", type=int, default=300)
+    ap.add_argument("--budget-gb", type=float, default=8.0)
+    args = ap.parse_args(argv)
+    srv = Server(budget_bytes=int(args.budget_gb * (1 << 30)))
+    make_sessions(srv, args.sessions, args.lo, args.hi)
+    torch.cuda.synchronize()
+    srv.start()
+    try:
+        poisson_load(srv, 20.0, 10)  # warm-up
+        for q in args.qps:
+            lat, wall, bs = poisson_load(srv, q, args.requests)
+            lat = np.sort(np.asarray(lat))
+            print(json.dumps({"qps_offered": q, "requests": len(lat), "throughput_rps": len(lat) / wall,
+                              "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                              "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                              "mean_batch": float(np.mean(bs))}), flush=True)
+    finally:
+        srv.stop()
+
+
+if __name__ == "__main__":
+    main()

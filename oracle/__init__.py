@@ -197,14 +197,19 @@ def reference_module():
     global _ref_mod
     if _ref_mod is not None:
         return _ref_mod
-    if not os.path.isdir(_REF_DIR) or not any(
-            f.startswith("FastGeodisCpp") and f.endswith(".so") for f in os.listdir(_REF_DIR)):
+    if not os.path.isdir(_REF_DIR):
         return None
+    libs = [f for f in os.listdir(_REF_DIR) if f.startswith("FastGeodisCpp") and f.endswith(".so")]
+    if not libs:
+        return None
+    import importlib.util
+
     import torch  # noqa: F401  (loads libtorch for the extension)
 
-    if _REF_DIR not in sys.path:
-        sys.path.insert(0, _REF_DIR)
-    import FastGeodisCpp  # type: ignore
-
-    _ref_mod = FastGeodisCpp
+    # loaded by path, never through sys.modules, so it cannot be confused
+    # with this repository's own FastGeodisCpp module
+    spec = importlib.util.spec_from_file_location("FastGeodisCpp", os.path.join(_REF_DIR, libs[0]))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    _ref_mod = mod
     return _ref_mod

@@ -1,0 +1,114 @@
+"""Batched form: many uneven 2D/3D instances in the same launches.
+
+Each instance's output must equal the single-instance call with the same
+arguments bit for bit (and therefore the oracle), whatever else shares the
+launches: other sizes, channel counts, operations, iteration counts.
+"""
+import numpy as np
+import pytest
+import torch
+
+import fastgeodis_amd as FG
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def bits_equal(a, b):
+    return torch.equal(a.contiguous().view(torch.int32), b.contiguous().view(torch.int32))
+
+
+def rand_instance(rng, nd, C, shape, op, lamb, it, spacing=None):
+    img = torch.from_numpy(rng.random((1, C, *shape), dtype=np.float32)).cuda()
+    if op == "unsigned":
+        m = np.ones((1, 1, *shape), np.float32)
+        for _ in range(int(rng.integers(1, 6))):
+            m[(0, 0) + tuple(int(rng.integers(0, s)) for s in shape)] = 0
+    else:
+        m = (rng.random((1, 1, *shape)) > 0.6).astype(np.float32)
+    mask = torch.from_numpy(m).cuda()
+    return FG.Instance(img, mask, 1e10, lamb, iter=it, spacing=spacing, op=op, theta=0.1)
+
+
+def single(inst):
+    i, m = inst.image, inst.softmask
+    if inst.image.dim() == 4:
+        if inst.op == "unsigned":
+            return FG.generalised_geodesic2d(i, m, inst.v, inst.lamb, inst.iter)
+        if inst.op == "signed":
+            return FG.signed_generalised_geodesic2d(i, m, inst.v, inst.lamb, inst.iter)
+        return FG.GSF2d(i, m, inst.theta, inst.v, inst.lamb, inst.iter)
+    if inst.op == "unsigned":
+        return FG.generalised_geodesic3d(i, m, inst.spacing, inst.v, inst.lamb, inst.iter)
+    if inst.op == "signed":
+        return FG.signed_generalised_geodesic3d(i, m, inst.spacing, inst.v, inst.lamb, inst.iter)
+    return FG.GSF3d(i, m, inst.theta, inst.spacing, inst.v, inst.lamb, inst.iter)
+
+
+def test_mixed_batch_equals_single_calls(device):
+    rng = np.random.default_rng(42)
+    insts = []
+    for k in range(10):
+        shape = (int(rng.integers(20, 200)), int(rng.integers(20, 200)))
+        insts.append(rand_instance(rng, 2, int(rng.choice([1, 3])), shape,
+                                   ["unsigned", "signed", "gsf"][k % 3], float(rng.choice([0.0, 0.5, 1.0])),
+                                   int(rng.integers(1, 4))))
+    for k in range(6):
+        shape = tuple(int(x) for x in rng.integers(8, 90, size=3))
+        sp = [float(x) for x in rng.uniform(0.5, 3.0, size=3)]
+        insts.append(rand_instance(rng, 3, int(rng.choice([1, 2])), shape,
+                                   ["unsigned", "signed", "gsf"][k % 3], float(rng.choice([0.3, 1.0])),
+                                   int(rng.integers(1, 4)), sp))
+    print("batch of", len(insts), flush=True)
+    outs = FG.geodesic_batch(insts)
+    torch.cuda.synchronize()
+    for k, (inst, out) in enumerate(zip(insts, outs)):
+        alone = single(inst)
+        assert bits_equal(out, alone), f"instance {k} ({inst.op}, {tuple(inst.image.shape)}) differs"
+
+
+def test_batch_matches_oracle_small(device):
+    rng = np.random.default_rng(7)
+    insts, want = [], []
+    for k in range(6):
+        if k % 2 == 0:
+            shape = (int(rng.integers(10, 80)), int(rng.integers(10, 80)))
+            inst = rand_instance(rng, 2, 1 + k % 3, shape, "unsigned", 0.5, 2)
+            want.append(oracle.py_geodesic2d(inst.image.cpu().numpy(), inst.softmask.cpu().numpy(), 1e10, 0.5, 2))
+        else:
+            shape = tuple(int(x) for x in rng.integers(5, 40, size=3))
+            inst = rand_instance(rng, 3, 1, shape, "signed", 1.0, 2, [1.5, 1.0, 0.7])
+            want.append(oracle.py_signed_geodesic3d(inst.image.cpu().numpy(), inst.softmask.cpu().numpy(),
+                                                    [1.5, 1.0, 0.7], 1e10, 1.0, 2))
+        insts.append(inst)
+    outs = FG.geodesic_batch(insts)
+    for out, w in zip(outs, want):
+        o = out.cpu().numpy()
+        assert np.array_equal(o.view(np.uint32), w.view(np.uint32))
+
+
+def test_batch_tuning_invariance(device):
+    """The same batch under different halo depths and workgroup shapes."""
+    rng = np.random.default_rng(3)
+    insts = [rand_instance(rng, 3, 1, (40, 130, 90), "unsigned", 1.0, 2, [1.0, 1.0, 1.0]),
+             rand_instance(rng, 2, 3, (300, 257), "unsigned", 0.5, 2),
+             rand_instance(rng, 3, 3, (33, 70, 71), "signed", 0.5, 2, [2.0, 1.0, 1.0])]
+    old = FG.get_tuning()
+    try:
+        FG.set_tuning(1, 4, 0, 0)
+        ref = FG.geodesic_batch(insts)
+        for e, k, s in [(1, 2, 1), (1, 6, 3), (0, 4, 0), (1, 3, 2)]:
+            FG.set_tuning(e, k, 0, s)
+            outs = FG.geodesic_batch(insts)
+            for a, b in zip(outs, ref):
+                assert bits_equal(a, b)
+    finally:
+        FG.set_tuning(**old)
+
+
+def test_empty_and_validation(device):
+    assert FG.geodesic_batch([]) == []
+    x = torch.rand(1, 1, 8, 8, device="cuda")
+    bad = FG.Instance(x, torch.rand(1, 1, 8, 9, device="cuda"), 1e10, 1.0)
+    with pytest.raises(ValueError):
+        FG.geodesic_batch([bad])
