@@ -100,6 +100,9 @@ def main():
     ap.add_argument("--iters", type=int, default=4)
     ap.add_argument("--halo", type=int, default=None, help="override the halo depth K")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibration-copy-mib", type=int, default=0,
+                    help="after the timed region, run one plain device copy of this many MiB "
+                         "(reference point for rocprofv3 byte counters)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -119,7 +122,7 @@ def main():
         sys.exit(2)
     if args.halo is not None:
         t = FG.get_tuning()
-        FG.set_tuning(t["engine"], args.halo, t["max_groups"])
+        FG.set_tuning(t["engine"], args.halo, t["max_groups"], t["shape"])
     tuning = FG.get_tuning()
 
     S = args.size
@@ -157,9 +160,19 @@ def main():
     traffic = read_traffic(os.path.join(ROOT, "profiles", "traffic_sweep_512.md"))
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "sweep_tiled_kernel<4,8,1,lamb1,fast>", "launches": launches,
+                "kernel": "sweep_tiled_kernel (C=1, lamb=1 fast path)", "launches": launches,
                 "mean_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": bytes_per_launch}
     whole_gbs = updates * 12 / (ms_per_step * 1e-3) / 1e9
+
+    if args.calibration_copy_mib > 0:
+        n = args.calibration_copy_mib * (1 << 20) // 4
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] calibration copy of {args.calibration_copy_mib} MiB done")
+        del src, dst
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -185,7 +198,7 @@ def main():
         "config": {"workload": f"generalised_geodesic3d {S}^3 C=1 lamb=1.0 iter={args.iters} "
                                "spacing=[1,1,1] v=1e10, 12-blob field, 16 seeds",
                    "shape": [1, 1, S, S, S], "iter": args.iters, "engine": tuning["engine"],
-                   "halo": tuning["halo"]},
+                   "halo": tuning["halo"], "tile_shape": tuning["shape"]},
         "whole_call_alg_GBps": round(whole_gbs, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -177,11 +177,13 @@ def geodesic_batch(instances: Sequence[Instance]) -> List[torch.Tensor]:
     )
 
 
-def set_tuning(engine: int = 1, halo: int = 4, max_groups: int = 0, shape: int = 0) -> None:
-    """Engine (0 per-slice, 1 persistent tiled), halo depth K, workgroup cap
-    (0 = occupancy limit) and 3D plane workgroup shape (0 auto, 1 = 4 waves x
-    8 rows, 2 = 8 x 4, 3 = 16 x 2, 4 = 8 x 4 with a deeper prefetch ring) used
-    by later calls on this host thread."""
+def set_tuning(engine: int = 1, halo: int = 0, max_groups: int = 0, shape: int = 0) -> None:
+    """Engine (0 per-slice, 1 persistent tiled), halo depth K (slices between
+    neighbour exchanges; 0 = automatic: the deepest halo whose tiles fit one
+    per compute unit), workgroup cap (0 = occupancy limit) and 3D plane
+    workgroup shape (0 auto, 1 = 4 waves x 8 rows, 2 = 8 x 4, 3 = 16 x 2,
+    4 = 8 x 4 with a deeper prefetch ring) used by later calls on this host
+    thread."""
     FastGeodisCpp.set_tuning(int(engine), int(halo), int(max_groups), int(shape))
 
 

@@ -252,6 +252,13 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   const int ow = min(nE, oe0 + TE) - oe0;
   const int lane_src = e - oe0;
 
+  // The block loop below advances in groups of P steps and exchanges halos
+  // every K steps; it stays within the sweep only when K is a positive
+  // multiple of P.  A launch that breaks this is refused before any store.
+  if (K <= 0 || K % P != 0) {
+    if (threadIdx.x == 0) __hip_atomic_store(status, FG_EINVAL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
+  }
   const int nsteps = nA - 1;
   if (nsteps <= 0) return;
   const int a_first = (dir > 0) ? 1 : nA - 2;
@@ -613,7 +620,10 @@ __global__ void fill_i32_kernel(int *p, int v) { *p = v; }
 // ---------------------------------------------------------------------------
 
 thread_local std::string g_err;
-thread_local fg_tuning g_tune = {1, 4, 0, 0};
+thread_local fg_tuning g_tune = {1, 0, 0, 0};
+
+// Requested halo depth: 0 = automatic (per plane), otherwise 1..kMaxK.
+int current_halo() { return std::max(0, std::min(g_tune.halo, kMaxK)); }
 
 // Per-launch timing of tiled sweeps (fg_profile_enable / fg_profile_collect).
 struct ProfRec {
@@ -673,6 +683,7 @@ struct Xform {
 // Tiled plan for one sweep plane.
 struct Plan {
   int NW, R;
+  int P;  // slab ring size of the kernel that runs this plan
   int K;  // halo depth actually used: the requested depth rounded up to a multiple of P
   int nTB, nTE, TB, TE;
   bool ok;
@@ -686,15 +697,17 @@ struct Shape {
 // (one row); the others tile 3D planes (selected by fg_tuning.shape).
 constexpr Shape kShapes[] = {{1, 1, 4, 4}, {4, 8, 2, 2}, {8, 4, 3, 2}, {16, 2, 3, 3}, {8, 4, 5, 2}};
 constexpr int kNumShapes = 5;
-thread_local int g_shape = 2;
+constexpr int kAutoShape = 4;  // fastest on 512^2 planes (deepest prefetch that keeps K = 5)
+thread_local int g_shape = kAutoShape;
 
-Plan plan_plane(int nB, int nE, int Kreq, int C) {
+Plan plan_fixed(int nB, int nE, int Kreq, int C) {
   Plan p{};
   p.ok = false;
   const Shape s = (nB == 1) ? kShapes[0] : kShapes[g_shape];
   p.NW = s.NW;
   p.R = s.R;
   const int P = C <= 2 ? s.P12 : s.P34;
+  p.P = P;
   const int K = ((Kreq + P - 1) / P) * P;
   p.K = K;
   const int WR = s.NW * s.R;
@@ -718,6 +731,24 @@ Plan plan_plane(int nB, int nE, int Kreq, int C) {
   }
   p.ok = true;
   return p;
+}
+
+// Plan of one sweep plane.  Kreq > 0: that halo depth, rounded up to a
+// multiple of the slab ring.  Kreq == 0 (automatic): the deepest halo
+// (fewest exchanges) whose tiles still fit one per compute unit, as a tile
+// shares its CU with no other and every tile waits for its neighbours at
+// each exchange; if even the shallowest halo needs more tiles, that one.
+Plan plan_plane(int nB, int nE, int Kreq, int C, int tile_budget) {
+  if (Kreq > 0) return plan_fixed(nB, nE, Kreq, C);
+  const Plan first = plan_fixed(nB, nE, 1, C);
+  Plan best = first;
+  for (int K = first.P; K <= kMaxK; K += first.P) {
+    const Plan q = plan_fixed(nB, nE, K, C);
+    if (!q.ok) break;
+    if ((long long)q.nTB * q.nTE > tile_budget) break;
+    best = q;
+  }
+  return best;
 }
 
 // (nB, nE) of each pass's sweep plane: 0 = D, 1 = H (2D: rows), 2 = W (2D: columns)
@@ -760,20 +791,17 @@ const void *pick_kernel_nr(int C, bool lamb1, bool fast) {
   return nullptr;
 }
 
-const void *pick_kernel(int NW, int R, int C, bool lamb1, bool fast) {
-  for (const Shape &s : kShapes) {
-    if (s.NW != NW || s.R != R) continue;
-    const int P = C <= 2 ? s.P12 : s.P34;
+// The kernel compiled for exactly this (NW, R, P) and channel count, or null.
+const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast) {
 #define FG_S(nw, r, p, clo, chi) \
-    if (NW == nw && R == r && P == p && C >= clo && C <= chi) return pick_kernel_nr<nw, r, p, clo, chi>(C, lamb1, fast);
-    FG_S(1, 1, 4, 1, 4)
-    FG_S(4, 8, 2, 1, 4)
-    FG_S(8, 4, 3, 1, 2)
-    FG_S(8, 4, 5, 1, 2)
-    FG_S(8, 4, 2, 3, 4)
-    FG_S(16, 2, 3, 1, 4)
+  if (NW == nw && R == r && P == p && C >= clo && C <= chi) return pick_kernel_nr<nw, r, p, clo, chi>(C, lamb1, fast);
+  FG_S(1, 1, 4, 1, 4)
+  FG_S(4, 8, 2, 1, 4)
+  FG_S(8, 4, 3, 1, 2)
+  FG_S(8, 4, 5, 1, 2)
+  FG_S(8, 4, 2, 3, 4)
+  FG_S(16, 2, 3, 1, 4)
 #undef FG_S
-  }
   return nullptr;
 }
 
@@ -910,16 +938,18 @@ struct Runner {
     }
   }
 
-  int record_tiled(std::vector<SweepJob> &jobs, int dir, int NW, int R, int C, bool lamb1, bool fast) {
+  int record_tiled(std::vector<SweepJob> &jobs, int dir, int NW, int R, int P, int C, bool lamb1, bool fast) {
     std::vector<SweepDesc> descs;
     long long halo_used = 0;
-    int tiles = 0, max_blocks = 0, kk = K;
+    int tiles = 0, max_blocks = 0, kk = -1;
     double alg = 0;
     for (auto &j : jobs) {
       SweepDesc s;
       fill_desc(*j.x, j.pass, dir, s);
       alg += (double)(s.nA - 1) * s.nB * s.nE * (4.0 * s.C + 8.0);
-      Plan p = plan_plane(s.nB, s.nE, K, s.C);
+      Plan p = plan_plane(s.nB, s.nE, K, s.C, dev.cus);
+      if (!p.ok || p.NW != NW || p.R != R || p.P != P || p.K % P != 0 || (kk >= 0 && kk != p.K))
+        return fail(FG_EINVAL, "internal: sweep plan does not match the selected kernel");
       s.nTB = p.nTB; s.nTE = p.nTE; s.TB = p.TB; s.TE = p.TE;
       s.tile0 = tiles;
       s.halo0 = halo_used;
@@ -938,7 +968,8 @@ struct Runner {
     }
     Cmd c;
     c.type = Cmd::TILED;
-    c.kern = pick_kernel(NW, R, C, lamb1, fast);
+    c.kern = pick_kernel(NW, R, P, C, lamb1, fast);
+    if (!c.kern) return fail(FG_EINVAL, "no tiled kernel for this shape");
     c.grid = tiles;
     c.block = NW * 64;
     c.off = rec.put(descs.data(), descs.size() * sizeof(SweepDesc));
@@ -1009,28 +1040,29 @@ struct Runner {
   // Record one pass (forward then backward) for the active transforms.
   int sweep_pass(std::vector<Xform *> &active, int pass) {
     for (int dir : {+1, -1}) {
-      struct G { int NW, R, C; bool l1, fa; std::vector<SweepJob> jobs; };
+      struct G { int NW, R, P, K, C; bool l1, fa; std::vector<SweepJob> jobs; };
       std::vector<G> groups;
       for (auto *x : active) {
         int nA, nB, nE;
         pass_plane(*x, pass, nA, nB, nE);
         if (nA <= 1) continue;
-        Plan p = plan_plane(nB, nE, K, x->C);
+        Plan p = plan_plane(nB, nE, K, x->C, dev.cus);
         if (engine == 0 || !p.ok || x->C > 4) {
           record_per_slice(*x, pass, dir);
           continue;
         }
         bool found = false;
         for (auto &g : groups)
-          if (g.NW == p.NW && g.R == p.R && g.C == x->C && g.l1 == x->lamb1 && g.fa == x->fast) {
+          if (g.NW == p.NW && g.R == p.R && g.P == p.P && g.K == p.K && g.C == x->C && g.l1 == x->lamb1 &&
+              g.fa == x->fast) {
             g.jobs.push_back({x, pass});
             found = true;
             break;
           }
-        if (!found) groups.push_back({p.NW, p.R, x->C, x->lamb1, x->fast, {{x, pass}}});
+        if (!found) groups.push_back({p.NW, p.R, p.P, p.K, x->C, x->lamb1, x->fast, {{x, pass}}});
       }
       for (auto &g : groups) {
-        int rc = record_group(g.jobs, dir, g.NW, g.R, g.C, g.l1, g.fa);
+        int rc = record_group(g.jobs, dir, g.NW, g.R, g.P, g.C, g.l1, g.fa);
         if (rc) return rc;
       }
     }
@@ -1038,8 +1070,8 @@ struct Runner {
   }
 
   // A group is split into launches whose tiles can all be co-resident.
-  int record_group(std::vector<SweepJob> &jobs, int dir, int NW, int R, int C, bool l1, bool fa) {
-    const void *kern = pick_kernel(NW, R, C, l1, fa);
+  int record_group(std::vector<SweepJob> &jobs, int dir, int NW, int R, int P, int C, bool l1, bool fa) {
+    const void *kern = pick_kernel(NW, R, P, C, l1, fa);
     if (!kern) return fail(FG_EINVAL, "no tiled kernel for this shape");
     long long cap = 0;
     int rc = coresident_groups(kern, NW * 64, dev, cap);
@@ -1050,21 +1082,21 @@ struct Runner {
     for (auto &j : jobs) {
       int nA, nB, nE;
       pass_plane(*j.x, j.pass, nA, nB, nE);
-      Plan p = plan_plane(nB, nE, K, j.x->C);
+      Plan p = plan_plane(nB, nE, K, j.x->C, dev.cus);
       const long long nt = (long long)p.nTB * p.nTE;
       if (nt > cap) {  // one plane alone cannot be co-resident: per-slice engine
         record_per_slice(*j.x, j.pass, dir);
         continue;
       }
       if (tiles + nt > cap) {
-        if ((rc = record_tiled(chunk, dir, NW, R, C, l1, fa))) return rc;
+        if ((rc = record_tiled(chunk, dir, NW, R, P, C, l1, fa))) return rc;
         chunk.clear();
         tiles = 0;
       }
       chunk.push_back(j);
       tiles += nt;
     }
-    if (!chunk.empty()) return record_tiled(chunk, dir, NW, R, C, l1, fa);
+    if (!chunk.empty()) return record_tiled(chunk, dir, NW, R, P, C, l1, fa);
     return FG_OK;
   }
 
@@ -1208,9 +1240,11 @@ size_t halo_entries_for(const std::vector<OpSpec> &ops, int K) {
     for (int p = 0; p < 3; ++p) {
       int nA, nB, nE;
       pass_plane(x, p, nA, nB, nE);
-      Plan pl = plan_plane(nB, nE, K, o.C);
-      if (!pl.ok) continue;
-      worst = std::max(worst, (size_t)pl.nTB * pl.nTE * 2 * pl.TB * pl.TE);
+      for (int k = (K > 0 ? K : 1); k <= (K > 0 ? K : kMaxK); ++k) {
+        Plan pl = plan_fixed(nB, nE, k, o.C);
+        if (!pl.ok) continue;
+        worst = std::max(worst, (size_t)pl.nTB * pl.nTE * 2 * pl.TB * pl.TE);
+      }
     }
     total += worst * (size_t)n_dist(o.op);
   }
@@ -1284,7 +1318,7 @@ int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream) 
     int rc = validate(o);
     if (rc) return rc;
   }
-  const int K = std::max(1, std::min(g_tune.halo, kMaxK));
+  const int K = current_halo();
   const size_t need = workspace_for(ops, K);
   if (!ws || wsb < need)
     return fail(FG_EWORKSPACE, "workspace too small: need " + std::to_string(need) + " bytes");
@@ -1446,7 +1480,6 @@ int run_single(OpSpec o, void *ws, size_t wsb, void *stream) {
   return run_ops(ops, ws, wsb, (hipStream_t)stream);
 }
 
-int current_halo() { return std::max(1, std::min(g_tune.halo, kMaxK)); }
 
 }  // namespace
 
@@ -1551,11 +1584,11 @@ int fg_geodesic_batch_f32(const fg_instance *inst, int n, void *workspace, size_
 int fg_tuning_set(const fg_tuning *t) {
   if (!t) return fail(FG_EINVAL, "null tuning");
   if (t->engine < 0 || t->engine > 1) return fail(FG_EINVAL, "engine must be 0 or 1");
-  if (t->halo < 1 || t->halo > kMaxK) return fail(FG_EINVAL, "halo must be in [1, 16]");
+  if (t->halo < 0 || t->halo > kMaxK) return fail(FG_EINVAL, "halo must be 0 (automatic) or in [1, 16]");
   if (t->max_groups < 0) return fail(FG_EINVAL, "max_groups must be >= 0");
   if (t->shape < 0 || t->shape >= kNumShapes) return fail(FG_EINVAL, "shape must be in [0, 4]");
   g_tune = *t;
-  g_shape = t->shape == 0 ? 2 : t->shape;
+  g_shape = t->shape == 0 ? kAutoShape : t->shape;
   return FG_OK;
 }
 
@@ -1599,6 +1632,7 @@ int fg_status(const void *workspace, void *stream) {
   e = hipMemcpy(&st, workspace, sizeof(int), hipMemcpyDeviceToHost);
   if (e != hipSuccess) return fail(FG_EHIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
   if (st == FG_EWATCHDOG) return fail(FG_EWATCHDOG, "a tile's halo wait expired (watchdog)");
+  if (st == FG_EINVAL) return fail(FG_EINVAL, "a sweep launch was refused: halo depth not a multiple of its slab ring");
   return FG_OK;
 }
 

@@ -77,13 +77,19 @@ class Server:
         self.budget = int(budget_bytes)
         self.max_batch = int(max_batch)
         self.v, self.lamb, self.iterations = float(v), float(lamb), int(iterations)
-        self.device = torch.device(device)
+        dev = torch.device(device)
+        if dev.type != "cuda":
+            raise ValueError("the serving loop runs on a HIP device")
+        if dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.device = dev
         self.inflight_limit = max(1, int(inflight))
         self.sessions: Dict[int, Session] = {}
         self._queue: collections.deque = collections.deque()
         self._cv = threading.Condition()
         self._stop = False
         self._thread: Optional[threading.Thread] = None
+        self._error: Optional[BaseException] = None
         self._stream = torch.cuda.Stream(device=self.device)
         self.batches: List[int] = []
         self.rejected = 0
@@ -113,6 +119,8 @@ class Server:
             raise ValueError("request larger than the HBM budget")
         fut: Future = Future()
         with self._cv:
+            if self._error is not None:
+                raise RuntimeError("the serving loop has failed") from self._error
             self._queue.append((click, fut))
             self._cv.notify()
         return fut
@@ -171,6 +179,20 @@ class Server:
 
     def _run(self):
         pending = collections.deque()  # (batch, outs, event, snaps)
+        try:
+            self._loop(pending)
+        except BaseException as exc:  # fail every outstanding request instead of hanging it
+            with self._cv:
+                self._error = exc
+                waiting = [f for _, f in self._queue]
+                self._queue.clear()
+            for b, *_ in pending:
+                waiting.extend(f for _, f in b)
+            for f in waiting:
+                if not f.done():
+                    f.set_exception(exc)
+
+    def _loop(self, pending):
         torch.cuda.set_device(self.device)
         while True:
             with self._cv:
@@ -180,7 +202,12 @@ class Server:
                     return
                 batch = self._admit() if len(pending) < self.inflight_limit else []
             if batch:
-                outs, ev, snaps = self._launch(batch)
+                try:
+                    outs, ev, snaps = self._launch(batch)
+                except BaseException as exc:
+                    for _, f in batch:
+                        f.set_exception(exc)
+                    raise
                 pending.append((batch, outs, ev, snaps))
                 self.batches.append(len(batch))
                 continue

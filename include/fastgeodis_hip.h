@@ -130,14 +130,16 @@ int fg_geodesic_batch_f32(const fg_instance *inst, int n, void *workspace,
 
 /* Tile/launch parameters of the persistent sweep engine.  engine 0 is the
  * one-launch-per-slice engine, 1 the persistent tiled engine (default).
- * halo is the trapezoid depth K (slices per neighbour exchange).  Values are
- * per host thread; fg_tuning_get reads them back. */
+ * halo is the trapezoid depth K (slices per neighbour exchange), rounded up
+ * to a multiple of the kernel's slab ring; 0 (default) picks, per sweep plane,
+ * the deepest halo whose tiles fit one per compute unit.  Values are per host
+ * thread; fg_tuning_get reads them back. */
 typedef struct fg_tuning {
   int engine;
   int halo;
   int max_groups; /* cap on persistent workgroups (0 = occupancy limit) */
-  int shape;      /* 3D plane workgroup shape (waves x rows / slab ring): 0 auto,
-                     1 = 4x8/2, 2 = 8x4/3, 3 = 16x2/3, 4 = 8x4/5 */
+  int shape;      /* 3D plane workgroup shape (waves x rows / slab ring): 0 auto
+                     (= 4), 1 = 4x8/2, 2 = 8x4/3, 3 = 16x2/3, 4 = 8x4/5 */
 } fg_tuning;
 int fg_tuning_set(const fg_tuning *t);
 int fg_tuning_get(fg_tuning *t);

@@ -65,6 +65,27 @@ def test_workspace_query(lib):
     assert big >= 2 * 64 ** 3 * 4
 
 
+def test_default_tuning_is_automatic():
+    import threading
+    got = {}
+    t = threading.Thread(target=lambda: got.update(FG.get_tuning()))  # fresh host thread
+    t.start(); t.join()
+    assert got == {"engine": 1, "halo": 0, "max_groups": 0, "shape": 0}
+
+
+def test_auto_halo_workspace_covers_fixed_depths():
+    """The automatic halo may pick any depth, so its workspace bounds every fixed one."""
+    old = FG.get_tuning()
+    try:
+        sizes = {}
+        for k in range(0, 17):
+            FG.set_tuning(1, k, 0, 0)
+            sizes[k] = FG.FastGeodisCpp.workspace_bytes(0, 1, 96, 300, 200)
+        assert all(sizes[0] >= sizes[k] for k in range(1, 17))
+    finally:
+        FG.set_tuning(**old)
+
+
 def test_tuning_roundtrip():
     old = FG.get_tuning()
     try:
@@ -74,8 +95,10 @@ def test_tuning_roundtrip():
             FG.set_tuning(1, 4, 0, 5)
         with pytest.raises(ValueError):
             FG.set_tuning(2, 4, 0)
+        FG.set_tuning(1, 0, 0)  # automatic halo depth
+        assert FG.get_tuning()["halo"] == 0
         with pytest.raises(ValueError):
-            FG.set_tuning(1, 0, 0)
+            FG.set_tuning(1, -1, 0)
         with pytest.raises(ValueError):
             FG.set_tuning(1, 17, 0)
     finally:

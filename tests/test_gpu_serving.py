@@ -40,7 +40,7 @@ def test_each_result_equals_alone(device):
     srv.start()
     try:
         futs = [srv.submit(c) for c in clicks]  # all at once: they get batched
-        res = [f.result(timeout=120) for f in futs]
+        res = [f.result(timeout=60) for f in futs]
     finally:
         srv.stop()
     assert max(r.batch_size for r in res) > 1, "requests were never batched together"
@@ -59,7 +59,7 @@ def test_budget_limits_batches(device):
     srv.start()
     try:
         futs = [srv.submit(serving.Click(sid, [(1, 1, 1)], True)) for sid in range(8)]
-        res = [f.result(timeout=120) for f in futs]
+        res = [f.result(timeout=60) for f in futs]
     finally:
         srv.stop()
     assert max(srv.batches) <= 3
