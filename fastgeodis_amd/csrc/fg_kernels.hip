@@ -209,6 +209,8 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     unsigned long long *__restrict__ halo_base, int *__restrict__ status, int spin_limit) {
   __shared__ float edge_first[2][NW][64];
   __shared__ float edge_last[2][NW][64];
+  __shared__ float edgeI_first[2][NW > 1 ? NW : 1][CT][64];
+  __shared__ float edgeI_last[2][NW > 1 ? NW : 1][CT][64];
 
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index: uniform
@@ -282,15 +284,25 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   // consumed.  Values at pad positions (outside the plane) are never used:
   // a pad's own result is forced to +inf after every step, and its image
   // value only enters candidates whose distance operand is that +inf.
-  auto load_slab = [&](Slab<R, CT> &S, int a) {
+  // With several waves, image rows -1 and R of a slab are the neighbour
+  // waves' rows R-1 and 0: they arrive through LDS when the slab is consumed
+  // (see step), so only rows 0..R-1 are loaded.  The initial front and the
+  // one-wave kernel load all rows.
+  auto load_rows = [&](Slab<R, CT> &S, int a, int r0, int r1) {
     const gfloat *ia = img + (long long)a * sA;
     const gfloat *da = dist + (long long)a * sA;
 #pragma unroll
     for (int r = 0; r < R + 2; ++r) {
+      if (r >= r0 && r < r1) {
 #pragma unroll
-      for (int c = 0; c < CT; ++c) S.I[c][r] = ld_at(ia + (c * cs + roff[r]), ecb);
+        for (int c = 0; c < CT; ++c) S.I[c][r] = ld_at(ia + (c * cs + roff[r]), ecb);
+      }
       if (r >= 1 && r <= R) S.d[r - 1] = ld_at(da + roff[r], ecb);
     }
+  };
+  auto load_slab = [&](Slab<R, CT> &S, int a) {
+    if constexpr (NW > 1) load_rows(S, a, 1, R + 1);
+    else load_rows(S, a, 0, R + 2);
   };
 
   // per-row validity of this lane (pad positions hold +inf in the front) and
@@ -308,25 +320,34 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   // S[P - 1].  Slabs stay in fixed registers this way.
   float Fd[R];  // front distance rows 0..R-1
   Slab<R, CT> S[P];
-  load_slab(S[P - 1], a_first - dir);  // front slice: not modified by this sweep
+  load_rows(S[P - 1], a_first - dir, 0, R + 2);  // front slice: not modified by this sweep
 #pragma unroll
   for (int r = 0; r < R; ++r) Fd[r] = pvalid[r] ? S[P - 1].d[r] : kInf;
 #pragma unroll
   for (int p = 0; p < P - 1; ++p) load_slab(S[p], min(max(a_first + p * dir, 0), nA - 1));
 
-  // Publishes this tile's own values of the current front and reloads the
-  // halo positions of the window from the neighbours' publications.  The
-  // prefetch of the next slice is issued between the halo loads and the wait
-  // for them, so that wait does not also wait for the new prefetch.
+  // Publishes the tile's own front values that neighbours read and reloads
+  // the halo positions of the window from the neighbours' publications.
+  //   published: own positions within K of a tile edge that has a neighbour
+  //   reloaded:  non-own window positions within K of the own region (the
+  //              only ones whose values reach own positions in K steps)
+  // A reloaded position's source tile publishes it: it lies within K of that
+  // tile's edge facing this tile.  The prefetch of the next slice is issued
+  // between the halo loads and the wait for them, so that wait does not also
+  // wait for the new prefetch; pending rows are polled together, so one
+  // memory round trip serves all of them.
   auto exchange = [&](int blk, Slab<R, CT> &pf_slab, int pf_step) {
     const unsigned tag = tag_base + (unsigned)blk;
     const long long hslot = (long long)(blk & 1) * tile_slots;
     gu64 *pub = halo + (long long)lt * 2 * tile_slots + hslot;
     const int own_w = e_own1 - e_own0;
+    const bool lane_band = (te > 0 && e - e_own0 < K) || (te < nTE - 1 && e_own1 - 1 - e < K);
+    const int lane_dist = max(max(e_own0 - e, e - (e_own1 - 1)), 0);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
       const int b = rb[r + 1];
-      if (b >= b_own0 && b < b_own1 && lane_own)
+      const bool row_band = (tb > 0 && b - b_own0 < K) || (tb < nTB - 1 && b_own1 - 1 - b < K);
+      if (b >= b_own0 && b < b_own1 && lane_own && (row_band || lane_band))
         __hip_atomic_store(pub + (long long)(b - b_own0) * own_w + lane_pub, pack(Fd[r], tag), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -337,7 +358,8 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     for (int r = 0; r < R; ++r) {
       const int b = rb[r + 1];
       const bool row_own = b >= b_own0 && b < b_own1;
-      need[r] = rvalid[r + 1] && lane_valid && !(row_own && lane_own);
+      const int row_dist = max(max(b_own0 - b, b - (b_own1 - 1)), 0);
+      need[r] = rvalid[r + 1] && lane_valid && !(row_own && lane_own) && max(row_dist, lane_dist) <= K;
       const int bq = min(max(b, 0), nB - 1);
       const int otb = bq / TB;  // uniform
       src[r] = halo + (long long)(otb * nTE + ote) * 2 * tile_slots + hslot + (long long)(bq - otb * TB) * ow +
@@ -346,20 +368,24 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
       if (need[r]) x[r] = __hip_atomic_load(src[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     load_slab(pf_slab, min(max(a_first + pf_step * dir, 0), nA - 1));
+    int spins = 0;
+    while (true) {
+      bool waiting = false;
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      if (need[r]) {
-        int spins = 0;
-        while ((unsigned)(x[r] >> 32) != tag) {
-          if (++spins > spin_limit) {
-            __hip_atomic_store(status, FG_EWATCHDOG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-          x[r] = __hip_atomic_load(src[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        Fd[r] = __uint_as_float((unsigned)x[r]);
+      for (int r = 0; r < R; ++r) waiting = waiting || (need[r] && (unsigned)(x[r] >> 32) != tag);
+      if (!waiting) break;
+      if (++spins > spin_limit) {
+        __hip_atomic_store(status, FG_EWATCHDOG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
       }
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        if (need[r] && (unsigned)(x[r] >> 32) != tag)
+          x[r] = __hip_atomic_load(src[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+#pragma unroll
+    for (int r = 0; r < R; ++r)
+      if (need[r]) Fd[r] = __uint_as_float((unsigned)x[r]);
   };
 
   // One slice: Sl holds the new slice, Fr the previous one (front image).
@@ -367,17 +393,29 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   auto step = [&](Slab<R, CT> &Sl, Slab<R, CT> &Fr, int s, bool prefetch) {
     const int a = a_first + s * dir;
     const int slot = s & 1;
-    // wave-boundary front rows through LDS
-    // (branch-free: the edge waves read a valid slot and select +inf)
+    // wave-boundary front rows through LDS, and the new slice's image rows
+    // -1 and R (the front image of the next step) from the neighbour waves
+    // (branch-free: the edge waves read their own slot and select +inf for
+    // the distance; their outer image rows then only meet +inf candidates)
     float top = kInf, bot = kInf;
     if constexpr (NW > 1) {
       edge_first[slot][w][lane] = Fd[0];
       edge_last[slot][w][lane] = Fd[R - 1];
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        edgeI_first[slot][w][c][lane] = Sl.I[c][1];
+        edgeI_last[slot][w][c][lane] = Sl.I[c][R];
+      }
       __syncthreads();
       const float t = edge_last[slot][max(w - 1, 0)][lane];
       const float u = edge_first[slot][min(w + 1, NW - 1)][lane];
       top = (w > 0) ? t : kInf;
       bot = (w < NW - 1) ? u : kInf;
+#pragma unroll
+      for (int c = 0; c < CT; ++c) {
+        Sl.I[c][0] = edgeI_last[slot][max(w - 1, 0)][c][lane];
+        Sl.I[c][R + 1] = edgeI_first[slot][min(w + 1, NW - 1)][c][lane];
+      }
     }
     float nd[R];
 #pragma unroll
