@@ -179,21 +179,9 @@ __device__ __forceinline__ float relax(const float (&dq)[3], const float (&Iq)[3
 // Tiled persistent sweep kernel
 // ---------------------------------------------------------------------------
 
-// Global-address-space views: loads and stores then use SGPR base + VGPR
-// offset addressing instead of flat addressing.
-typedef __attribute__((address_space(1))) float gfloat;
+// Global-address-space view of the halo publications (64-bit atomics use
+// SGPR base + VGPR offset addressing instead of flat addressing).
 typedef __attribute__((address_space(1))) unsigned long long gu64;
-
-// Loads/stores at a uniform base plus a 32-bit per-lane byte offset, which
-// maps onto SGPR-base + VGPR-offset addressing.
-typedef __attribute__((address_space(1))) const char gcbyte;
-typedef __attribute__((address_space(1))) char gbyte;
-__device__ __forceinline__ float ld_at(const gfloat *base, unsigned byte_off) {
-  return *(const gfloat *)((gcbyte *)base + byte_off);
-}
-__device__ __forceinline__ void st_at(gfloat *base, unsigned byte_off, float v) {
-  *(gfloat *)((gbyte *)base + byte_off) = v;
-}
 
 template <int R, int CT>
 struct Slab {  // one prefetched slice: image rows -1..R, distance rows 0..R-1
@@ -223,8 +211,6 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     if (descs[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
   }
   const SweepDesc &Dd = descs[lo];
-  const gfloat *img = (const gfloat *)Dd.img;
-  gfloat *dist = (gfloat *)Dd.d;
   gu64 *halo = (gu64 *)halo_base + Dd.halo0;
   const long long cs = Dd.cs, sA = Dd.sA, sB = Dd.sB;
   const int nA = Dd.nA, nB = Dd.nB, nE = Dd.nE, dir = Dd.dir;
@@ -265,19 +251,25 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   if (nsteps <= 0) return;
   const int a_first = (dir > 0) ? 1 : nA - 2;
 
-  // uniform row bookkeeping: rows -1..R relative to this wave
+  // Row bookkeeping: rows -1..R relative to this wave.  rofs[r] is the byte
+  // offset of this lane's element of row r (clamped into the plane) within
+  // a slice; the host runs this kernel only on arrays of less than 4 GiB per
+  // channel, so every offset in a slice fits 32 bits and all loads and
+  // stores use a per-slice buffer resource plus this loop-invariant offset.
   int rb[R + 2];
   bool rvalid[R + 2];
-  long long roff[R + 2];
-  bool rows_all_valid = true;
+  unsigned rofs[R + 2];
 #pragma unroll
   for (int r = 0; r < R + 2; ++r) {
     rb[r] = bw0 + w * R + r - 1;
     rvalid[r] = rb[r] >= 0 && rb[r] < nB;
-    rows_all_valid = rows_all_valid && rvalid[r];
-    roff[r] = (long long)min(max(rb[r], 0), nB - 1) * sB;
+    rofs[r] = (unsigned)((long long)min(max(rb[r], 0), nB - 1) * sB * 4) + ecb;
   }
-  (void)rows_all_valid;
+  // buffer resource of slice a of an array (raw, byte-addressed); offsets at
+  // or beyond the range are out of bounds and stores there are dropped
+  auto slice_rsrc = [&](const float *base, int a, unsigned range = 0xFFFFFFF0u) {
+    return __builtin_amdgcn_make_buffer_rsrc((void *)(base + (long long)a * sA), 0, range, 0x00020000);
+  };
 
   // Prefetch loads are raw loads at clamped coordinates (always in bounds) and
   // carry no masking, so no instruction waits on them until the slab is
@@ -289,15 +281,19 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   // (see step), so only rows 0..R-1 are loaded.  The initial front and the
   // one-wave kernel load all rows.
   auto load_rows = [&](Slab<R, CT> &S, int a, int r0, int r1) {
-    const gfloat *ia = img + (long long)a * sA;
-    const gfloat *da = dist + (long long)a * sA;
+    const __amdgpu_buffer_rsrc_t rd = slice_rsrc(Dd.d, a);
+    __amdgpu_buffer_rsrc_t ri[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) ri[c] = slice_rsrc(Dd.img + c * cs, a);
 #pragma unroll
     for (int r = 0; r < R + 2; ++r) {
       if (r >= r0 && r < r1) {
 #pragma unroll
-        for (int c = 0; c < CT; ++c) S.I[c][r] = ld_at(ia + (c * cs + roff[r]), ecb);
+        for (int c = 0; c < CT; ++c)
+          S.I[c][r] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ri[c], (int)rofs[r], 0, 0));
       }
-      if (r >= 1 && r <= R) S.d[r - 1] = ld_at(da + roff[r], ecb);
+      if (r >= 1 && r <= R)
+        S.d[r - 1] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rd, (int)rofs[r], 0, 0));
     }
   };
   auto load_slab = [&](Slab<R, CT> &S, int a) {
@@ -351,6 +347,8 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
         __hip_atomic_store(pub + (long long)(b - b_own0) * own_w + lane_pub, pack(Fd[r], tag), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
+    // Only lanes that need a halo value load (measured faster than letting
+    // every lane load some in-bounds line and selecting afterwards).
     const gu64 *src[R];
     bool need[R];
     unsigned long long x[R];
@@ -362,8 +360,9 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
       need[r] = rvalid[r + 1] && lane_valid && !(row_own && lane_own) && max(row_dist, lane_dist) <= K;
       const int bq = min(max(b, 0), nB - 1);
       const int otb = bq / TB;  // uniform
-      src[r] = halo + (long long)(otb * nTE + ote) * 2 * tile_slots + hslot + (long long)(bq - otb * TB) * ow +
-               lane_src;
+      const gu64 *from = halo + (long long)(otb * nTE + ote) * 2 * tile_slots + hslot +
+                         (long long)(bq - otb * TB) * ow + lane_src;
+      src[r] = from;
       x[r] = 0;
       if (need[r]) x[r] = __hip_atomic_load(src[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -391,7 +390,10 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   // One slice: Sl holds the new slice, Fr the previous one (front image).
   // Afterwards Fr is refilled with the slice of step s + P - 1.
   auto step = [&](Slab<R, CT> &Sl, Slab<R, CT> &Fr, int s, bool prefetch) {
-    const int a = a_first + s * dir;
+    // steps past the end of the sweep (the last block is padded to K steps)
+    // compute on clamped slices and store nothing
+    const bool live = s < nsteps;
+    const int a = min(max(a_first + s * dir, 0), nA - 1);
     const int slot = s & 1;
     // wave-boundary front rows through LDS, and the new slice's image rows
     // -1 and R (the front image of the next step) from the neighbour waves
@@ -437,11 +439,11 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     }
     // own outputs: raw buffer stores, non-own lanes and rows get an
     // out-of-range offset and the hardware drops their writes (no branch)
-    float *da = Dd.d + (long long)a * sA;
+    // a padded step's resource has an empty range: all its stores are dropped
+    const __amdgpu_buffer_rsrc_t rs = slice_rsrc(Dd.d, a, live ? 0xFFFFFFF0u : 0u);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(da + roff[r + 1], 0, 0x7FFFFFF0, 0x00020000);
-      const unsigned off = (row_own[r] && lane_own) ? ecb : 0xFFFFFFFFu;
+      const unsigned off = (row_own[r] && lane_own) ? rofs[r + 1] : 0xFFFFFFFFu;
       __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(nd[r]), rs, (int)off, 0, 0);
     }
 #pragma unroll
@@ -451,33 +453,27 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     if (prefetch) load_slab(Fr, min(max(a + (P - 1) * dir, 0), nA - 1));
   };
 
-  // Blocks of K steps (K is a multiple of P, so every block starts at step
-  // s with s % P == 0); halo exchange between blocks.  No branch separates
-  // the steps of a group of P, which keeps the waits on prefetched slabs
-  // precise.
-  int s = 0;
-  while (true) {
-    const int blk_end = min(s + K, nsteps);
-    if (blk_end == nsteps) {
-      for (; s + P <= nsteps; s += P) {
+  // Blocks of exactly K steps (K a multiple of P, so a block starts at a
+  // step s with s % P == 0 and slab p always sits in S[p]); the last block is
+  // padded with steps that store nothing.  Every path into the group loop
+  // and the block loop holds the ring in the same registers, so the waits
+  // on prefetched slabs stay precise (no block boundary drains the loads).
+  // The last step of a block defers its prefetch into the exchange.
+  const int groups = K / P;
+  const int nblocks = (nsteps + K - 1) / K;
+  for (int blk = 0;; ++blk) {
+    const int s0 = blk * K;
+    for (int g = 0; g < groups - 1; ++g) {
 #pragma unroll
-        for (int p = 0; p < P; ++p) step(S[p], S[(p + P - 1) % P], s + p, true);
-      }
-#pragma unroll
-      for (int p = 0; p < P; ++p)
-        if (s + p < nsteps) step(S[p], S[(p + P - 1) % P], s + p, true);
-      break;
+      for (int p = 0; p < P; ++p) step(S[p], S[(p + P - 1) % P], s0 + g * P + p, true);
     }
-    for (; s + P < blk_end; s += P) {
+    const int sl = s0 + K - P;
 #pragma unroll
-      for (int p = 0; p < P; ++p) step(S[p], S[(p + P - 1) % P], s + p, true);
-    }
-#pragma unroll
-    for (int p = 0; p < P - 1; ++p) step(S[p], S[(p + P - 1) % P], s + p, true);
-    step(S[P - 1], S[P - 2], s + P - 1, false);
-    s += P;
-    // the deferred prefetch of step s - 1 goes into S[P - 2]: slice of step s - 1 + P - 1
-    exchange(s / K, S[P - 2], s + P - 2);
+    for (int p = 0; p < P - 1; ++p) step(S[p], S[(p + P - 1) % P], sl + p, true);
+    step(S[P - 1], S[P - 2], sl + P - 1, false);
+    if (blk + 1 >= nblocks) break;
+    // deferred prefetch of step s0 + K - 1: the slice of step s0 + K + P - 2
+    exchange(blk + 1, S[P - 2], s0 + K + P - 2);
   }
 }
 
@@ -958,6 +954,15 @@ struct Runner {
     for (int k = 0; k < 9; ++k) s.e1[k] = x.e1[pass][k];
   }
 
+  // The tiled kernel addresses elements of a slice with 32-bit byte offsets
+  // from the slice start (buffer resources with a range of 0xFFFFFFF0 bytes).
+  bool slice_offsets_fit(const Xform &x, int pass) {
+    SweepDesc s;
+    fill_desc(x, pass, 1, s);
+    const long long span = ((long long)(s.nB - 1) * s.sB + s.nE) * 4;
+    return span < 0xFFFFFFF0ll;
+  }
+
   void record_per_slice(const Xform &x, int pass, int dir) {
     SweepDesc s;
     fill_desc(x, pass, dir, s);
@@ -1085,7 +1090,7 @@ struct Runner {
         pass_plane(*x, pass, nA, nB, nE);
         if (nA <= 1) continue;
         Plan p = plan_plane(nB, nE, K, x->C, dev.cus);
-        if (engine == 0 || !p.ok || x->C > 4) {
+        if (engine == 0 || !p.ok || x->C > 4 || !slice_offsets_fit(*x, pass)) {
           record_per_slice(*x, pass, dir);
           continue;
         }
