@@ -19,7 +19,11 @@ Prints ONE JSON line on stdout; progress goes to stderr.
              WRITE_SIZE (profiles/traffic_sweep_512.md), or null
   cpu_baseline  the reference's own OpenMP CPU path (compiled from its
              sources into oracle/_ref) or, without it, the C restatement
-             (oracle/), on a bounded sample of the same workload
+             (oracle/), on the same volume and call, run once
+  click_latency  p50/p99 of the serving loop (64 resident sessions, sides
+             96-320, signed iter 2) at 50 and 200 offered clicks/s
+  batched_vs_per_call  one batched call against a loop of plain calls on 256
+             uneven 2D + 32 uneven 3D instances (outputs compared bit for bit)
 """
 from __future__ import annotations
 
@@ -68,13 +72,16 @@ def cpu_baseline(img_gpu, mask_gpu, iters: int):
     sys.path.insert(0, ROOT)
     import oracle
 
-    sample_d = int(os.environ.get("FG_CPU_SAMPLE_DEPTH", "256"))
+    sample_d = int(os.environ.get("FG_CPU_SAMPLE_DEPTH", str(img_gpu.shape[2])))
     img = img_gpu[:, :, :sample_d].contiguous().cpu()
     mask = mask_gpu[:, :, :sample_d].contiguous().cpu()
     n = img[0, 0].numel()
     threads = int(os.environ.get("OMP_NUM_THREADS", str(os.cpu_count() or 1)))
     ref = oracle.reference_module()
-    sample = f"{sample_d}x512x512 slab of the same volume, lamb=1.0, iter={iters}, one run"
+    S = img_gpu.shape[3]
+    sample = (f"the same {sample_d}x{S}x{S} volume and call (lamb=1.0, iter={iters}), one run"
+              if sample_d == img_gpu.shape[2] else
+              f"{sample_d}x{S}x{S} slab of the same volume, lamb=1.0, iter={iters}, one run")
     if ref is not None:
         kind = "reference"
         t0 = time.perf_counter()
@@ -100,6 +107,10 @@ def main():
     ap.add_argument("--iters", type=int, default=4)
     ap.add_argument("--halo", type=int, default=None, help="override the halo depth K")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true",
+                    help="skip the click-latency sample (64 resident sessions, Poisson arrivals)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-vs-per-call comparison on the uneven 2D/3D workload")
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="after the timed region, run one plain device copy of this many MiB "
                          "(reference point for rocprofv3 byte counters)")
@@ -177,8 +188,27 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline sample")
-        cpu = cpu_baseline(img, mask, 2)
+        cpu = cpu_baseline(img, mask, args.iters)
         log(f"[bench] cpu baseline {cpu['value'] / 1e9:.3f} Gvu/s ({cpu['kind']})")
+
+    serving_res = None
+    if not args.no_serving:
+        from fastgeodis_amd import workloads
+
+        log("[bench] click latency: 64 resident sessions, signed iter 2, Poisson arrivals")
+        c5 = workloads.run_c5(rates=(50, 200), requests=300)
+        serving_res = {"workload": c5["workload"], "mean_session_voxels": c5["mean_session_voxels"],
+                       "rows": [{k: (round(v, 2) if isinstance(v, float) else v) for k, v in r.items()}
+                                for r in c5["rows"]]}
+    batch_res = None
+    if not args.no_batch:
+        from fastgeodis_amd import workloads
+
+        log("[bench] batched vs per-call: 256 uneven 2D + 32 uneven 3D instances")
+        c4 = workloads.run_c4(reps=2)
+        batch_res = {"workload": c4["workload"], "identical": c4["identical"],
+                     "per_call_ms": round(c4["loop_ms"], 2), "batched_ms": round(c4["batch_ms"], 2),
+                     "speedup": round(c4["speedup"], 2)}
 
     line = {
         "metric": "voxel-updates/s + % HBM peak, 512^3 geodesic3d; p50/p99 click latency",
@@ -202,6 +232,8 @@ def main():
         "whole_call_alg_GBps": round(whole_gbs, 1),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "click_latency": serving_res,
+        "batched_vs_per_call": batch_res,
     }
     print(json.dumps(line), flush=True)
 

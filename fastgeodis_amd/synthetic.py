@@ -74,3 +74,46 @@ def config1(device="cpu"):
     shape = (512, 512)
     img = blob_field(shape, 8, (20.0, 80.0), 1, device)[None, None]
     return img, seed_mask(shape, 4, 1, device)
+
+
+def config3(device="cuda", batch: int = 32, size: int = 1024):
+    """B RGB images (per-channel blob fields normalised to [0, 1]) with disc
+    masks (0 inside a disc of radius U[50, 300] px at a uniform centre, 1
+    outside) and 2 % of pixels flipped (default_rng(3))."""
+    rng = np.random.default_rng(3)
+    images, masks = [], []
+    yy, xx = torch.meshgrid(torch.arange(size, device=device, dtype=torch.float32),
+                            torch.arange(size, device=device, dtype=torch.float32), indexing="ij")
+    for b in range(batch):
+        chans = [blob_field((size, size), 8, (20.0, 80.0), 3000 + 3 * b + c, device, normalise=True)
+                 for c in range(3)]
+        images.append(torch.stack(chans)[None])
+        r = rng.uniform(50, 300)
+        cy, cx = rng.uniform(0, size), rng.uniform(0, size)
+        m = (((yy - cy) ** 2 + (xx - cx) ** 2) > r * r).float()
+        g = torch.Generator(device=device)
+        g.manual_seed(int(rng.integers(0, 2 ** 31)))
+        flip = torch.rand((size, size), generator=g, device=device) < 0.02
+        m = torch.where(flip, 1.0 - m, m)
+        masks.append(m[None, None])
+    return images, masks
+
+
+def config4(device="cuda", n2d: int = 256, n3d: int = 32):
+    """Uneven batch (default_rng(4)): n2d 2D instances with H, W log-uniform in
+    [64, 2048] and C in {1, 3}, and n3d 3D volumes with sides uniform in
+    [64, 256] and per-axis spacing U[0.5, 3.0]; blob-field images, 1-32 seeds.
+    Returns a list of (image, mask, spacing or None)."""
+    rng = np.random.default_rng(4)
+    out = []
+    for k in range(n2d):
+        H, W = (int(round(np.exp(rng.uniform(np.log(64), np.log(2048))))) for _ in range(2))
+        C = int(rng.choice([1, 3]))
+        img = torch.stack([blob_field((H, W), 8, (8.0, 80.0), 40000 + 4 * k + c, device) for c in range(C)])
+        out.append((img[None], seed_mask((H, W), int(rng.integers(1, 33)), 40000 + k, device), None))
+    for k in range(n3d):
+        shape = tuple(int(x) for x in rng.integers(64, 257, size=3))
+        sp = [float(x) for x in rng.uniform(0.5, 3.0, size=3)]
+        img = blob_field(shape, 12, (8.0, 40.0), 50000 + k, device)[None, None]
+        out.append((img, seed_mask(shape, int(rng.integers(1, 33)), 50000 + k, device), sp))
+    return out
