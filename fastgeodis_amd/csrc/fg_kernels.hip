@@ -195,10 +195,13 @@ template <int NW, int R, int P, int CT, bool LAMB1, bool FAST>
 __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     const SweepDesc *__restrict__ descs, int ndesc, int K, unsigned tag_base,
     unsigned long long *__restrict__ halo_base, int *__restrict__ status, int spin_limit) {
-  __shared__ float edge_first[2][NW][64];
-  __shared__ float edge_last[2][NW][64];
-  __shared__ float edgeI_first[2][NW > 1 ? NW : 1][CT][64];
-  __shared__ float edgeI_last[2][NW > 1 ? NW : 1][CT][64];
+  // LDS slot = ring position of the step (a compile-time index in the
+  // unrolled loops, so edge addresses are immediate offsets)
+  constexpr int NWL = NW > 1 ? NW : 1;
+  __shared__ float edge_first[P][NWL][64];
+  __shared__ float edge_last[P][NWL][64];
+  __shared__ float edgeI_first[P][NWL][CT][64];
+  __shared__ float edgeI_last[P][NWL][CT][64];
 
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index: uniform
@@ -389,12 +392,11 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
 
   // One slice: Sl holds the new slice, Fr the previous one (front image).
   // Afterwards Fr is refilled with the slice of step s + P - 1.
-  auto step = [&](Slab<R, CT> &Sl, Slab<R, CT> &Fr, int s, bool prefetch) {
+  auto step = [&](Slab<R, CT> &Sl, Slab<R, CT> &Fr, int s, int slot, bool prefetch) {
     // steps past the end of the sweep (the last block is padded to K steps)
     // compute on clamped slices and store nothing
     const bool live = s < nsteps;
     const int a = min(max(a_first + s * dir, 0), nA - 1);
-    const int slot = s & 1;
     // wave-boundary front rows through LDS, and the new slice's image rows
     // -1 and R (the front image of the next step) from the neighbour waves
     // (branch-free: the edge waves read their own slot and select +inf for
@@ -465,12 +467,12 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     const int s0 = blk * K;
     for (int g = 0; g < groups - 1; ++g) {
 #pragma unroll
-      for (int p = 0; p < P; ++p) step(S[p], S[(p + P - 1) % P], s0 + g * P + p, true);
+      for (int p = 0; p < P; ++p) step(S[p], S[(p + P - 1) % P], s0 + g * P + p, p, true);
     }
     const int sl = s0 + K - P;
 #pragma unroll
-    for (int p = 0; p < P - 1; ++p) step(S[p], S[(p + P - 1) % P], sl + p, true);
-    step(S[P - 1], S[P - 2], sl + P - 1, false);
+    for (int p = 0; p < P - 1; ++p) step(S[p], S[(p + P - 1) % P], sl + p, p, true);
+    step(S[P - 1], S[P - 2], sl + P - 1, P - 1, false);
     if (blk + 1 >= nblocks) break;
     // deferred prefetch of step s0 + K - 1: the slice of step s0 + K + P - 2
     exchange(blk + 1, S[P - 2], s0 + K + P - 2);
