@@ -73,6 +73,8 @@ struct SweepDesc {
   int TB, TE;    // own tile extent
   int tile0;     // first tile of this descriptor within its launch
   long long halo0;  // first halo entry of this descriptor
+  float *dxo;       // transposed-output kernels: output array, element (a, b, e) at
+  long long sXe, sXa;  //   dxo[e * sXe + a * sXa + b] (b contiguous)
 };
 
 struct XposeDesc {  // out[c][w][h][d] = in[c][d][h][w]
@@ -191,7 +193,11 @@ struct Slab {  // one prefetched slice: image rows -1..R, distance rows 0..R-1
 
 // NW waves per workgroup, R window rows per wave (register rows), P slices
 // prefetched ahead, CT image channels.
-template <int NW, int R, int P, int CT, bool LAMB1, bool FAST>
+// XST: instead of writing slice a in the sweep's layout, write it to Dd.dxo
+// with B contiguous (the H pass writes the (W, H, D) layout of the W pass),
+// staging each slice's window through LDS so each store covers a run of
+// consecutive B.
+template <int NW, int R, int P, int CT, bool LAMB1, bool FAST, bool XST = false>
 __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     const SweepDesc *__restrict__ descs, int ndesc, int K, unsigned tag_base,
     unsigned long long *__restrict__ halo_base, int *__restrict__ status, int spin_limit) {
@@ -202,6 +208,9 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   __shared__ float edge_last[P][NWL][64];
   __shared__ float edgeI_first[P][NWL][CT][64];
   __shared__ float edgeI_last[P][NWL][CT][64];
+  constexpr int NWR = NW * R;  // window rows
+  static_assert(!XST || (NW > 1 && 64 % NWR == 0), "transposed output needs a multi-wave window of 16/32/64 rows");
+  __shared__ float xtile[XST ? P : 1][XST ? NWR : 1][65];
 
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index: uniform
@@ -274,6 +283,36 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     return __builtin_amdgcn_make_buffer_rsrc((void *)(base + (long long)a * sA), 0, range, 0x00020000);
   };
 
+  // Transposed output: wave w moves window columns (w * R + k) * (64 / NWR)
+  // + lane / NWR, rows lane % NWR, for k < R; the byte offset of each element
+  // in its output slice (out-of-range for positions this tile does not own).
+  unsigned xoff[XST ? R : 1];
+  if constexpr (XST) {
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      const int col = (w * R + k) * (64 / NWR) + lane / NWR, row = lane % NWR;
+      const int eg = ew0 + col, bg = bw0 + row;
+      const bool own = eg >= e_own0 && eg < e_own1 && bg >= b_own0 && bg < b_own1;
+      xoff[k] = own ? (unsigned)(((long long)eg * Dd.sXe + bg) * 4) : 0xFFFFFFFFu;
+    }
+  }
+  // stores the staged window of step sp (LDS slot ps) to the output; step -1
+  // is the sweep's first slice, which the sweep itself never changes but the
+  // output array must also hold
+  auto xflush = [&](int ps, int sp) {
+    if constexpr (XST) {
+      const bool live = sp >= -1 && sp < nsteps;
+      const int ap = min(max(a_first + sp * dir, 0), nA - 1);
+      const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc(
+          (void *)(Dd.dxo + (long long)ap * Dd.sXa), 0, live ? 0xFFFFFFF0u : 0u, 0x00020000);
+#pragma unroll
+      for (int k = 0; k < R; ++k) {
+        const int col = (w * R + k) * (64 / NWR) + lane / NWR, row = lane % NWR;
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(xtile[ps][row][col]), rx, (int)xoff[k], 0, 0);
+      }
+    }
+  };
+
   // Prefetch loads are raw loads at clamped coordinates (always in bounds) and
   // carry no masking, so no instruction waits on them until the slab is
   // consumed.  Values at pad positions (outside the plane) are never used:
@@ -322,6 +361,10 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   load_rows(S[P - 1], a_first - dir, 0, R + 2);  // front slice: not modified by this sweep
 #pragma unroll
   for (int r = 0; r < R; ++r) Fd[r] = pvalid[r] ? S[P - 1].d[r] : kInf;
+  if constexpr (XST) {  // stage the first slice as step -1 (ring slot P - 1)
+#pragma unroll
+    for (int r = 0; r < R; ++r) xtile[P - 1][w * R + r][lane] = Fd[r];
+  }
 #pragma unroll
   for (int p = 0; p < P - 1; ++p) load_slab(S[p], min(max(a_first + p * dir, 0), nA - 1));
 
@@ -420,6 +463,8 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
         Sl.I[c][0] = edgeI_last[slot][max(w - 1, 0)][c][lane];
         Sl.I[c][R + 1] = edgeI_first[slot][min(w + 1, NW - 1)][c][lane];
       }
+      // the previous step's staged window is complete (barrier above)
+      xflush((slot + P - 1) % P, s - 1);
     }
     float nd[R];
 #pragma unroll
@@ -441,12 +486,17 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     }
     // own outputs: raw buffer stores, non-own lanes and rows get an
     // out-of-range offset and the hardware drops their writes (no branch)
-    // a padded step's resource has an empty range: all its stores are dropped
-    const __amdgpu_buffer_rsrc_t rs = slice_rsrc(Dd.d, a, live ? 0xFFFFFFF0u : 0u);
+    if constexpr (XST) {
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const unsigned off = (row_own[r] && lane_own) ? rofs[r + 1] : 0xFFFFFFFFu;
-      __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(nd[r]), rs, (int)off, 0, 0);
+      for (int r = 0; r < R; ++r) xtile[slot][w * R + r][lane] = nd[r];
+    } else {
+      // a padded step's resource has an empty range: all its stores are dropped
+      const __amdgpu_buffer_rsrc_t rs = slice_rsrc(Dd.d, a, live ? 0xFFFFFFF0u : 0u);
+#pragma unroll
+      for (int r = 0; r < R; ++r) {
+        const unsigned off = (row_own[r] && lane_own) ? rofs[r + 1] : 0xFFFFFFFFu;
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(nd[r]), rs, (int)off, 0, 0);
+      }
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) Fd[r] = nd[r];
@@ -476,6 +526,10 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     if (blk + 1 >= nblocks) break;
     // deferred prefetch of step s0 + K - 1: the slice of step s0 + K + P - 2
     exchange(blk + 1, S[P - 2], s0 + K + P - 2);
+  }
+  if constexpr (XST) {
+    __syncthreads();
+    xflush(P - 1, nblocks * K - 1);  // the last step sits in slot P - 1
   }
 }
 
@@ -714,6 +768,7 @@ struct Xform {
   float e1[3][9];  // per pass: D, H, W (2D: [1] rows pass, [2] columns pass)
   bool lamb1, fast;
   int iters;
+  bool xst;  // this iteration's H-backward sweep writes dT directly (no transpose before the W pass)
 };
 
 // Tiled plan for one sweep plane.
@@ -801,17 +856,17 @@ void pass_plane(const Xform &x, int pass, int &nA, int &nB, int &nE) {
   else { nA = x.W; nB = x.H; nE = x.D; }
 }
 
-template <int NW, int R, int P, int CT, bool L1, bool FA>
+template <int NW, int R, int P, int CT, bool L1, bool FA, bool XS = false>
 const void *kernel_ptr() {
-  return reinterpret_cast<const void *>(&sweep_tiled_kernel<NW, R, P, CT, L1, FA>);
+  return reinterpret_cast<const void *>(&sweep_tiled_kernel<NW, R, P, CT, L1, FA, XS>);
 }
 
 // lamb1 implies fast (l_grad == 1 and l_eucl*LD == +0 are non-negative).
 // Only the channel counts a (shape, ring) pair serves are instantiated.
-template <int NW, int R, int P, int CT>
+template <int NW, int R, int P, int CT, bool XS = false>
 const void *pick_variant(bool lamb1, bool fast) {
-  if (lamb1) return kernel_ptr<NW, R, P, CT, true, true>();
-  return fast ? kernel_ptr<NW, R, P, CT, false, true>() : kernel_ptr<NW, R, P, CT, false, false>();
+  if (lamb1) return kernel_ptr<NW, R, P, CT, true, true, XS>();
+  return fast ? kernel_ptr<NW, R, P, CT, false, true, XS>() : kernel_ptr<NW, R, P, CT, false, false, XS>();
 }
 
 template <int NW, int R, int P, int CLO, int CHI>
@@ -828,7 +883,15 @@ const void *pick_kernel_nr(int C, bool lamb1, bool fast) {
 }
 
 // The kernel compiled for exactly this (NW, R, P) and channel count, or null.
-const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast) {
+// Transposed-output kernels exist for the default 3D shape and C <= 2.
+constexpr int kXstNW = 8, kXstR = 4, kXstP = 5, kXstMaxC = 2;
+const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast, bool xst = false) {
+  if (xst) {
+    if (NW != kXstNW || R != kXstR || P != kXstP) return nullptr;
+    if (C == 1) return pick_variant<kXstNW, kXstR, kXstP, 1, true>(lamb1, fast);
+    if (C == 2) return pick_variant<kXstNW, kXstR, kXstP, 2, true>(lamb1, fast);
+    return nullptr;
+  }
 #define FG_S(nw, r, p, clo, chi) \
   if (NW == nw && R == r && P == p && C >= clo && C <= chi) return pick_kernel_nr<nw, r, p, clo, chi>(C, lamb1, fast);
   FG_S(1, 1, 4, 1, 4)
@@ -934,7 +997,7 @@ struct Runner {
   unsigned tag_base = 0;
   Recorder rec;
 
-  void fill_desc(const Xform &x, int pass, int dir, SweepDesc &s) {
+  void fill_desc(const Xform &x, int pass, int dir, SweepDesc &s, bool xst = false) {
     std::memset(&s, 0, sizeof s);
     const long long N = (long long)x.D * x.H * x.W;
     s.C = x.C;
@@ -954,6 +1017,26 @@ struct Runner {
       s.img = x.imgT; s.d = x.dT; s.sA = (long long)x.H * x.D; s.sB = x.D;
     }
     for (int k = 0; k < 9; ++k) s.e1[k] = x.e1[pass][k];
+    if (xst) {  // H pass (A = H, B = D, E = W) written as (W, H, D)
+      s.dxo = x.dT;
+      s.sXe = (long long)s.nA * s.nB;
+      s.sXa = s.nB;
+    }
+  }
+
+  // Whether the H-backward sweep of this transform can write the transposed
+  // layout itself: 3D, tiled engine with the default shape and a
+  // transposed-output kernel for its channel count, and every output offset
+  // within 32 bits.
+  bool xstore_candidate(const Xform &x) {
+    if (x.is2d || engine == 0 || x.C > kXstMaxC) return false;
+    int nA, nB, nE;
+    pass_plane(x, 1, nA, nB, nE);
+    if (nA <= 1 || nB <= 1) return false;
+    const Plan p = plan_plane(nB, nE, K, x.C, dev.cus);
+    if (!p.ok || p.NW != kXstNW || p.R != kXstR || p.P != kXstP) return false;
+    if ((long long)x.D * x.H * x.W * 4 >= 0xFFFFFFF0ll) return false;
+    return slice_offsets_fit(x, 1);
   }
 
   // The tiled kernel addresses elements of a slice with 32-bit byte offsets
@@ -983,14 +1066,15 @@ struct Runner {
     }
   }
 
-  int record_tiled(std::vector<SweepJob> &jobs, int dir, int NW, int R, int P, int C, bool lamb1, bool fast) {
+  int record_tiled(std::vector<SweepJob> &jobs, int dir, int NW, int R, int P, int C, bool lamb1, bool fast,
+                   bool xs = false) {
     std::vector<SweepDesc> descs;
     long long halo_used = 0;
     int tiles = 0, max_blocks = 0, kk = -1;
     double alg = 0;
     for (auto &j : jobs) {
       SweepDesc s;
-      fill_desc(*j.x, j.pass, dir, s);
+      fill_desc(*j.x, j.pass, dir, s, xs);
       alg += (double)(s.nA - 1) * s.nB * s.nE * (4.0 * s.C + 8.0);
       Plan p = plan_plane(s.nB, s.nE, K, s.C, dev.cus);
       if (!p.ok || p.NW != NW || p.R != R || p.P != P || p.K % P != 0 || (kk >= 0 && kk != p.K))
@@ -1013,7 +1097,7 @@ struct Runner {
     }
     Cmd c;
     c.type = Cmd::TILED;
-    c.kern = pick_kernel(NW, R, P, C, lamb1, fast);
+    c.kern = pick_kernel(NW, R, P, C, lamb1, fast, xs);
     if (!c.kern) return fail(FG_EINVAL, "no tiled kernel for this shape");
     c.grid = tiles;
     c.block = NW * 64;
@@ -1085,7 +1169,7 @@ struct Runner {
   // Record one pass (forward then backward) for the active transforms.
   int sweep_pass(std::vector<Xform *> &active, int pass) {
     for (int dir : {+1, -1}) {
-      struct G { int NW, R, P, K, C; bool l1, fa; std::vector<SweepJob> jobs; };
+      struct G { int NW, R, P, K, C; bool l1, fa, xs; std::vector<SweepJob> jobs; };
       std::vector<G> groups;
       for (auto *x : active) {
         int nA, nB, nE;
@@ -1096,18 +1180,19 @@ struct Runner {
           record_per_slice(*x, pass, dir);
           continue;
         }
+        const bool xs = pass == 1 && dir < 0 && x->xst;
         bool found = false;
         for (auto &g : groups)
           if (g.NW == p.NW && g.R == p.R && g.P == p.P && g.K == p.K && g.C == x->C && g.l1 == x->lamb1 &&
-              g.fa == x->fast) {
+              g.fa == x->fast && g.xs == xs) {
             g.jobs.push_back({x, pass});
             found = true;
             break;
           }
-        if (!found) groups.push_back({p.NW, p.R, p.P, p.K, x->C, x->lamb1, x->fast, {{x, pass}}});
+        if (!found) groups.push_back({p.NW, p.R, p.P, p.K, x->C, x->lamb1, x->fast, xs, {{x, pass}}});
       }
       for (auto &g : groups) {
-        int rc = record_group(g.jobs, dir, g.NW, g.R, g.P, g.C, g.l1, g.fa);
+        int rc = record_group(g.jobs, dir, g.NW, g.R, g.P, g.C, g.l1, g.fa, g.xs);
         if (rc) return rc;
       }
     }
@@ -1115,8 +1200,9 @@ struct Runner {
   }
 
   // A group is split into launches whose tiles can all be co-resident.
-  int record_group(std::vector<SweepJob> &jobs, int dir, int NW, int R, int P, int C, bool l1, bool fa) {
-    const void *kern = pick_kernel(NW, R, P, C, l1, fa);
+  int record_group(std::vector<SweepJob> &jobs, int dir, int NW, int R, int P, int C, bool l1, bool fa,
+                   bool xs = false) {
+    const void *kern = pick_kernel(NW, R, P, C, l1, fa, xs);
     if (!kern) return fail(FG_EINVAL, "no tiled kernel for this shape");
     long long cap = 0;
     int rc = coresident_groups(kern, NW * 64, dev, cap);
@@ -1131,17 +1217,18 @@ struct Runner {
       const long long nt = (long long)p.nTB * p.nTE;
       if (nt > cap) {  // one plane alone cannot be co-resident: per-slice engine
         record_per_slice(*j.x, j.pass, dir);
+        j.x->xst = false;  // its output stays in the native layout
         continue;
       }
       if (tiles + nt > cap) {
-        if ((rc = record_tiled(chunk, dir, NW, R, P, C, l1, fa))) return rc;
+        if ((rc = record_tiled(chunk, dir, NW, R, P, C, l1, fa, xs))) return rc;
         chunk.clear();
         tiles = 0;
       }
       chunk.push_back(j);
       tiles += nt;
     }
-    if (!chunk.empty()) return record_tiled(chunk, dir, NW, R, P, C, l1, fa);
+    if (!chunk.empty()) return record_tiled(chunk, dir, NW, R, P, C, l1, fa, xs);
     return FG_OK;
   }
 
@@ -1153,13 +1240,14 @@ struct Runner {
       for (auto &x : xs)
         if (x.iters > it) active.push_back(&x);
       int rc;
+      for (auto *x : active) x->xst = xstore_candidate(*x);
       if ((rc = sweep_pass(active, 0))) return rc;
       if ((rc = sweep_pass(active, 1))) return rc;
       std::vector<XposeDesc> to, back;
       for (auto *x : active) {
         int D, H, W;
         native_dims(*x, D, H, W);
-        to.push_back(xd(x->d, x->dT, 1, D, H, W));
+        if (!x->xst) to.push_back(xd(x->d, x->dT, 1, D, H, W));
         back.push_back(xd(x->dT, x->d, 1, W, H, D));
       }
       record_xpose(to);
