@@ -599,6 +599,33 @@ __global__ __launch_bounds__(256) void xpose_kernel(const XposeDesc *__restrict_
   const long long cn = (long long)X.D * HW;
   const float *src = X.src + c * cn + (long long)h * X.W;
   float *dst = X.dst + c * cn + (long long)h * X.D;
+  if ((X.W & 3) == 0 && (X.D & 3) == 0 && d0 + kXT <= X.D && w0 + kXT <= X.W &&
+      (((uintptr_t)X.src | (uintptr_t)X.dst) & 15) == 0) {
+    // full tile, 16-byte rows: each thread moves float4s (16 threads per
+    // 256-byte row segment) and the LDS tile keeps its padded scalar layout
+    const int q = threadIdx.x & 15, r0 = threadIdx.x >> 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int r = r0 + 16 * k;
+      const float4 v = *reinterpret_cast<const float4 *>(src + (long long)(d0 + r) * HW + w0 + 4 * q);
+      tile[r][4 * q + 0] = v.x;
+      tile[r][4 * q + 1] = v.y;
+      tile[r][4 * q + 2] = v.z;
+      tile[r][4 * q + 3] = v.w;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int r = r0 + 16 * k;  // output row (w)
+      float4 v;
+      v.x = tile[4 * q + 0][r];
+      v.y = tile[4 * q + 1][r];
+      v.z = tile[4 * q + 2][r];
+      v.w = tile[4 * q + 3][r];
+      *reinterpret_cast<float4 *>(dst + (long long)(w0 + r) * HD + d0 + 4 * q) = v;
+    }
+    return;
+  }
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   for (int r = ty; r < kXT; r += 4) {
     const int d = d0 + r, w = w0 + tx;
