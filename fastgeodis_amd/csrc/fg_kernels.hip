@@ -1235,6 +1235,14 @@ struct Runner {
     int rc = coresident_groups(kern, NW * 64, dev, cap);
     if (rc) return rc;
     if (max_groups > 0) cap = std::min<long long>(cap, max_groups);
+    // A launch lasts as long as its longest sweep: pack planes of similar
+    // step counts together (longest first).
+    std::stable_sort(jobs.begin(), jobs.end(), [](const SweepJob &u, const SweepJob &v) {
+      int au, bu, eu, av, bv, ev;
+      pass_plane(*u.x, u.pass, au, bu, eu);
+      pass_plane(*v.x, v.pass, av, bv, ev);
+      return au > av;
+    });
     std::vector<SweepJob> chunk;
     long long tiles = 0;
     for (auto &j : jobs) {
