@@ -670,17 +670,46 @@ __device__ __forceinline__ float signed_closed(float v, float m) {
   return v * m - v * (1.0f - m);
 }
 
+__device__ __forceinline__ int ew_find(const EwDesc *__restrict__ descs, int ndesc, long long g) {
+  int lo = 0, hi = ndesc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (descs[mid].elem0 <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+__device__ __forceinline__ void ew_one(const EwDesc &E, long long i);
+
+// Each thread handles 4 consecutive elements; the unsigned initialisation
+// (every plain call) uses one 16-byte load and store when the 4 elements
+// belong to one descriptor and are aligned.
 __global__ __launch_bounds__(256) void ew_kernel(const EwDesc *__restrict__ descs, int ndesc,
                                                   long long total) {
-  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-       g += (long long)gridDim.x * blockDim.x) {
-    int lo = 0, hi = ndesc - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (descs[mid].elem0 <= g) lo = mid; else hi = mid - 1;
+  for (long long g0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * 4; g0 < total;
+       g0 += (long long)gridDim.x * blockDim.x * 4) {
+    const EwDesc &E = descs[ew_find(descs, ndesc, g0)];
+    const long long i0 = g0 - E.elem0;
+    if (E.kind == EW_INIT_UNSIGNED && i0 + 4 <= E.nelem &&
+        (((uintptr_t)(E.out + i0) | (uintptr_t)(E.mask + i0)) & 15) == 0) {
+      const float4 m = *reinterpret_cast<const float4 *>(E.mask + i0);
+      float4 o;
+      o.x = E.v * m.x;
+      o.y = E.v * m.y;
+      o.z = E.v * m.z;
+      o.w = E.v * m.w;
+      *reinterpret_cast<float4 *>(E.out + i0) = o;
+      continue;
     }
-    const EwDesc &E = descs[lo];
-    const long long i = g - E.elem0;
+    for (int j = 0; j < 4 && g0 + j < total; ++j) {
+      const EwDesc &F = descs[ew_find(descs, ndesc, g0 + j)];
+      ew_one(F, g0 + j - F.elem0);
+    }
+  }
+}
+
+__device__ __forceinline__ void ew_one(const EwDesc &E, long long i) {
+  {
     const float v = E.v;
     switch (E.kind) {
       case EW_INIT_UNSIGNED:
@@ -1161,7 +1190,7 @@ struct Runner {
     c.ndesc = (int)ds.size();
     c.total = total;
     c.block = 256;
-    c.grid = (int)std::max<long long>(1, std::min<long long>(ceil_div(total, 256), (long long)dev.cus * 16));
+    c.grid = (int)std::max<long long>(1, std::min<long long>(ceil_div(total, 1024), (long long)dev.cus * 16));
     rec.cmds.push_back(c);
   }
 
