@@ -1,0 +1,126 @@
+"""Empirical tuner for the persistent sweep engine.
+
+The engine's own defaults are analytic: the 8 x 4 workgroup with a 5-slice
+prefetch ring and, per plane, the deepest halo whose tiles fit one per
+compute unit.  For other problem sizes a different workgroup shape or halo
+depth can be faster; this module measures that directly: it builds a
+synthetic blob-field input of the requested size on the device, times the
+plain transform under each candidate ``(engine, halo, shape)`` (see
+:func:`fastgeodis_amd.set_tuning`), checks that every candidate returns the
+same bits, and keeps the fastest.  Results are cached per problem signature
+in memory and, when ``FASTGEODIS_AMD_TUNE_CACHE`` names a file, in that JSON
+file.
+
+    from fastgeodis_amd import autotune
+    best = autotune.tune((128, 256, 256), channels=1, spacing=(2.5, 1, 1))
+    autotune.apply(best)            # tuning for later calls on this thread
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+from typing import Dict, Iterable, List, Optional, Sequence, Tuple
+
+import torch
+
+from . import generalised_geodesic2d, generalised_geodesic3d, get_tuning, set_tuning
+
+# (engine, halo, shape): halo 0 = automatic depth, shape 0 = default
+DEFAULT_CANDIDATES_3D: Tuple[Tuple[int, int, int], ...] = (
+    (1, 0, 4), (1, 0, 2), (1, 0, 3), (1, 0, 1),
+    (1, 3, 2), (1, 6, 2), (1, 10, 4), (1, 8, 1),
+)
+DEFAULT_CANDIDATES_2D: Tuple[Tuple[int, int, int], ...] = (
+    (1, 0, 0), (1, 4, 0), (1, 8, 0), (1, 12, 0),
+)
+
+_cache: Dict[str, dict] = {}
+
+
+def signature(shape: Sequence[int], channels: int) -> str:
+    return f"{len(shape)}d:{'x'.join(str(int(s)) for s in shape)}:c{int(channels)}"
+
+
+def _cache_file() -> Optional[str]:
+    return os.environ.get("FASTGEODIS_AMD_TUNE_CACHE") or None
+
+
+def load_cache() -> Dict[str, dict]:
+    path = _cache_file()
+    if path and os.path.isfile(path):
+        with open(path) as f:
+            _cache.update(json.load(f))
+    return _cache
+
+
+def _save_cache() -> None:
+    path = _cache_file()
+    if path:
+        tmp = path + ".tmp"
+        with open(tmp, "w") as f:
+            json.dump(_cache, f, indent=1, sort_keys=True)
+        os.replace(tmp, path)
+
+
+def _input(shape: Sequence[int], channels: int, device):
+    from .synthetic import blob_field, seed_mask
+
+    chans = [blob_field(shape, 8, (8.0, 40.0), 9000 + c, device) for c in range(channels)]
+    img = torch.stack(chans)[None]
+    return img, seed_mask(shape, 8, 9000, device)
+
+
+def tune(shape: Sequence[int], channels: int = 1, spacing: Sequence[float] = (1.0, 1.0, 1.0),
+         lamb: float = 1.0, iters: int = 1, reps: int = 3,
+         candidates: Optional[Iterable[Tuple[int, int, int]]] = None, device="cuda",
+         use_cache: bool = True) -> dict:
+    """Measures candidate tunings for one problem size; returns
+    ``{"signature", "best": {"engine", "halo", "shape"}, "ms", "table"}``."""
+    shape = [int(s) for s in shape]
+    if len(shape) not in (2, 3):
+        raise ValueError("shape must be (H, W) or (D, H, W)")
+    sig = signature(shape, channels)
+    if use_cache and sig in load_cache():
+        return _cache[sig]
+    if candidates is None:
+        candidates = DEFAULT_CANDIDATES_3D if len(shape) == 3 else DEFAULT_CANDIDATES_2D
+    img, mask = _input(shape, channels, device)
+
+    def call():
+        if len(shape) == 2:
+            return generalised_geodesic2d(img, mask, 1e10, lamb, iters)
+        return generalised_geodesic3d(img, mask, list(spacing), 1e10, lamb, iters)
+
+    old = get_tuning()
+    table: List[dict] = []
+    ref = None
+    try:
+        for (engine, halo, shp) in candidates:
+            set_tuning(engine, halo, 0, shp)
+            out = call()
+            torch.cuda.synchronize()
+            if ref is None:
+                ref = out.clone()
+            elif not torch.equal(out.view(torch.int32), ref.view(torch.int32)):
+                raise RuntimeError(f"tuning {(engine, halo, shp)} changed the result")
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                call()
+            torch.cuda.synchronize()
+            table.append({"engine": engine, "halo": halo, "shape": shp,
+                          "ms": (time.perf_counter() - t0) * 1e3 / reps})
+    finally:
+        set_tuning(**old)
+    best = min(table, key=lambda r: r["ms"])
+    res = {"signature": sig, "best": {k: best[k] for k in ("engine", "halo", "shape")},
+           "ms": best["ms"], "table": table}
+    _cache[sig] = res
+    _save_cache()
+    return res
+
+
+def apply(result: dict) -> None:
+    """Makes a :func:`tune` result the tuning of later calls on this thread."""
+    b = result["best"]
+    set_tuning(int(b["engine"]), int(b["halo"]), 0, int(b["shape"]))
