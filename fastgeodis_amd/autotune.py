@@ -92,10 +92,12 @@ def tune(shape: Sequence[int], channels: int = 1, spacing: Sequence[float] = (1.
             return generalised_geodesic2d(img, mask, 1e10, lamb, iters)
         return generalised_geodesic3d(img, mask, list(spacing), 1e10, lamb, iters)
 
+    candidates = list(candidates)
     old = get_tuning()
     table: List[dict] = []
     ref = None
     try:
+        # first pass: every candidate once (warm-up, and the results must agree)
         for (engine, halo, shp) in candidates:
             set_tuning(engine, halo, 0, shp)
             out = call()
@@ -104,6 +106,11 @@ def tune(shape: Sequence[int], channels: int = 1, spacing: Sequence[float] = (1.
                 ref = out.clone()
             elif not torch.equal(out.view(torch.int32), ref.view(torch.int32)):
                 raise RuntimeError(f"tuning {(engine, halo, shp)} changed the result")
+        # second pass: timing
+        for (engine, halo, shp) in candidates:
+            set_tuning(engine, halo, 0, shp)
+            call()
+            torch.cuda.synchronize()
             t0 = time.perf_counter()
             for _ in range(reps):
                 call()
