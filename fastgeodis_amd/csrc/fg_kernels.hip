@@ -13,7 +13,10 @@
 //   D pass: A=D, B=H, E=W on the native (D,H,W) array
 //   H pass: A=H, B=D, E=W on the native array
 //   W pass: A=W, B=H, E=D on a transposed (W,H,D) copy (image transposed once
-//           per call, distance transposed in and out around the W pass pair)
+//           per call; the distance enters that layout through the H pass's
+//           backward sweep, which writes it directly (or, for 3-4 channels and
+//           non-default shapes, through a transpose) and leaves it through a
+//           transpose after the W pass pair)
 // B and E are the reference's in-plane axes in its neighbour order (B outer,
 // E inner), so the exact-order path visits candidates in the reference order.
 //
@@ -242,7 +245,6 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   const unsigned ec = (unsigned)min(max(e, 0), nE - 1);  // clamped: loads never leave the row
   const unsigned ecb = ec * 4u;                             // its byte offset within a row
   const bool lane_own = (e >= e_own0) && (e < e_own1);
-  const bool pad_lanes = (ew0 < 0) || (ew0 + 63 >= nE);
   const long long tile_slots = (long long)TB * TE;
   // halo publication slot of this lane (own lanes) and source tile column
   // of this lane (halo lanes)
