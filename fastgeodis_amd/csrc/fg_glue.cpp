@@ -125,7 +125,8 @@ torch::Tensor run_device(const Call &c, const torch::Tensor &image_in, const tor
     H = (int)image.size(3);
     W = (int)image.size(4);
   }
-  if (out.numel() == 0 || image.numel() == 0) return out;
+  if (out.numel() == 0) return out;
+  if (C == 0) throw std::invalid_argument("image must have at least one channel");
   const size_t wsb = fg_workspace_bytes(c.op, C, c.nd == 2 ? 0 : D, H, W);
   torch::Tensor ws = torch::empty({(int64_t)wsb}, image.options().dtype(torch::kUInt8));
   void *stream = (void *)c10::hip::getCurrentHIPStream(image.device().index()).stream();
@@ -246,6 +247,7 @@ std::vector<torch::Tensor> geodesic_batch(const std::vector<torch::Tensor> &imag
   const auto dev = images[0].device();
   if (!images[0].is_cuda()) throw std::invalid_argument("batched calls take HIP tensors");
   c10::DeviceGuard guard(dev);
+  std::vector<fg_instance> work;  // instances with a non-empty output
   for (size_t i = 0; i < n; ++i) {
     const int nd = (int)images[i].dim() - 2;
     if (nd != 2 && nd != 3) throw std::invalid_argument("instance " + std::to_string(i) + ": need 2D or 3D");
@@ -261,6 +263,8 @@ std::vector<torch::Tensor> geodesic_batch(const std::vector<torch::Tensor> &imag
     ims.push_back(images[i].contiguous());
     mks.push_back(masks[i].contiguous());
     outs.push_back(torch::empty_like(mks.back(), torch::MemoryFormat::Contiguous));
+    if (outs.back().numel() == 0) continue;  // nothing to compute, as in the single-instance call
+    if (ims[i].size(1) == 0) throw std::invalid_argument("instance " + std::to_string(i) + ": image must have at least one channel");
     fg_instance &f = inst[i];
     f.image = ims[i].data_ptr<float>();
     f.mask = mks[i].data_ptr<float>();
@@ -278,12 +282,14 @@ std::vector<torch::Tensor> geodesic_batch(const std::vector<torch::Tensor> &imag
     f.l_grad = (float)lgs[i];
     f.l_eucl = (float)les[i];
     f.theta = (float)thetas[i];
+    work.push_back(f);
   }
-  const size_t wsb = fg_batch_workspace_bytes(inst.data(), (int)n);
+  if (work.empty()) return outs;
+  const size_t wsb = fg_batch_workspace_bytes(work.data(), (int)work.size());
   if (wsb == 0) raise_for(FG_EINVAL);
   torch::Tensor ws = torch::empty({(int64_t)wsb}, ims[0].options().dtype(torch::kUInt8));
   void *stream = (void *)c10::hip::getCurrentHIPStream(dev.index()).stream();
-  raise_for(fg_geodesic_batch_f32(inst.data(), (int)n, ws.data_ptr(), wsb, stream));
+  raise_for(fg_geodesic_batch_f32(work.data(), (int)work.size(), ws.data_ptr(), wsb, stream));
   if (sync_check_enabled()) raise_for(fg_status(ws.data_ptr(), stream));
   return outs;
 }
@@ -322,8 +328,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("GSF2d", &GSF2d, "Geodesic Symmetric Filtering 2d");
   m.def("GSF3d", &GSF3d, "Geodesic Symmetric Filtering 3d");
   m.def("geodesic_batch", &geodesic_batch, "Batched raster-scan transforms of uneven sizes");
-  m.def("set_tuning", &set_tuning, "Set engine / halo depth / workgroup cap for this host thread");
-  m.def("get_tuning", &get_tuning, "Current (engine, halo, max_groups) of this host thread");
+  m.def("set_tuning", &set_tuning, "Set engine / halo depth / workgroup cap / 3D shape for this host thread");
+  m.def("get_tuning", &get_tuning, "Current (engine, halo, max_groups, shape) of this host thread");
   m.def("workspace_bytes", &workspace_bytes, "Workspace bytes of one call");
   m.def("abi_version", []() { return fg_abi_version(); });
   m.def("profile_enable", &profile_enable, "Bracket tiled sweep launches with HIP events");

@@ -112,3 +112,16 @@ def test_empty_and_validation(device):
     bad = FG.Instance(x, torch.rand(1, 1, 8, 9, device="cuda"), 1e10, 1.0)
     with pytest.raises(ValueError):
         FG.geodesic_batch([bad])
+
+
+def test_empty_instance_in_batch_and_zero_channel_image(device):
+    rng = np.random.default_rng(11)
+    full = rand_instance(rng, 2, 1, (40, 70), "unsigned", 1.0, 2)
+    empty = FG.Instance(torch.rand(1, 1, 0, 9, device="cuda"), torch.ones(1, 1, 0, 9, device="cuda"), 1e10, 1.0)
+    outs = FG.geodesic_batch([empty, full])
+    assert outs[0].shape == (1, 1, 0, 9)
+    assert bits_equal(outs[1], single(full))
+    # an image with no channels cannot define a gradient term: refused, never garbage
+    with pytest.raises(ValueError):
+        FG.generalised_geodesic2d(torch.rand(1, 0, 8, 8, device="cuda"), torch.ones(1, 1, 8, 8, device="cuda"),
+                                  1e10, 1.0, 2)
