@@ -181,9 +181,9 @@ def set_tuning(engine: int = 1, halo: int = 0, max_groups: int = 0, shape: int =
     """Engine (0 per-slice, 1 persistent tiled), halo depth K (slices between
     neighbour exchanges; 0 = automatic: the deepest halo whose tiles fit one
     per compute unit), workgroup cap (0 = occupancy limit) and 3D plane
-    workgroup shape (0 auto, 1 = 4 waves x 8 rows, 2 = 8 x 4, 3 = 16 x 2,
-    4 = 8 x 4 with a deeper prefetch ring) used by later calls on this host
-    thread."""
+    workgroup shape (0 auto = 5, 1 = 4 waves x 8 rows, 2 = 8 x 4, 3 = 16 x 2,
+    4 = 8 x 4 with a deeper prefetch ring, 5 = 16 x 2 with the deeper ring)
+    used by later calls on this host thread."""
     FastGeodisCpp.set_tuning(int(engine), int(halo), int(max_groups), int(shape))
 
 

@@ -851,9 +851,9 @@ struct Shape {
 };
 // Workgroup shapes with compiled kernels: index 0 serves 2D row sweeps
 // (one row); the others tile 3D planes (selected by fg_tuning.shape).
-constexpr Shape kShapes[] = {{1, 1, 4, 4}, {4, 8, 2, 2}, {8, 4, 3, 2}, {16, 2, 3, 3}, {8, 4, 5, 2}};
-constexpr int kNumShapes = 5;
-constexpr int kAutoShape = 4;  // fastest on 512^2 planes (deepest prefetch that keeps K = 5)
+constexpr Shape kShapes[] = {{1, 1, 4, 4}, {4, 8, 2, 2}, {8, 4, 3, 2}, {16, 2, 3, 3}, {8, 4, 5, 2}, {16, 2, 5, 3}};
+constexpr int kNumShapes = 6;
+constexpr int kAutoShape = 5;  // fastest on 512^2 planes: 16 x 2 with the deepest ring that keeps K = 5
 thread_local int g_shape = kAutoShape;
 
 Plan plan_fixed(int nB, int nE, int Kreq, int C) {
@@ -948,13 +948,14 @@ const void *pick_kernel_nr(int C, bool lamb1, bool fast) {
 }
 
 // The kernel compiled for exactly this (NW, R, P) and channel count, or null.
-// Transposed-output kernels exist for the default 3D shape and C <= 2.
-constexpr int kXstNW = 8, kXstR = 4, kXstP = 5, kXstMaxC = 2;
+// Transposed-output kernels exist for the two 5-slice-ring 3D shapes: 8 x 4
+// with 1-2 channels, 16 x 2 with 1 channel (2 channels would exceed the LDS).
+constexpr int kXstMaxC = 2;
 const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast, bool xst = false) {
   if (xst) {
-    if (NW != kXstNW || R != kXstR || P != kXstP) return nullptr;
-    if (C == 1) return pick_variant<kXstNW, kXstR, kXstP, 1, true>(lamb1, fast);
-    if (C == 2) return pick_variant<kXstNW, kXstR, kXstP, 2, true>(lamb1, fast);
+    if (NW == 8 && R == 4 && P == 5 && C == 1) return pick_variant<8, 4, 5, 1, true>(lamb1, fast);
+    if (NW == 8 && R == 4 && P == 5 && C == 2) return pick_variant<8, 4, 5, 2, true>(lamb1, fast);
+    if (NW == 16 && R == 2 && P == 5 && C == 1) return pick_variant<16, 2, 5, 1, true>(lamb1, fast);
     return nullptr;
   }
 #define FG_S(nw, r, p, clo, chi) \
@@ -963,6 +964,7 @@ const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast, bool
   FG_S(4, 8, 2, 1, 4)
   FG_S(8, 4, 3, 1, 2)
   FG_S(8, 4, 5, 1, 2)
+  FG_S(16, 2, 5, 1, 2)
   FG_S(8, 4, 2, 3, 4)
   FG_S(16, 2, 3, 1, 4)
 #undef FG_S
@@ -1099,7 +1101,7 @@ struct Runner {
     pass_plane(x, 1, nA, nB, nE);
     if (nA <= 1 || nB <= 1) return false;
     const Plan p = plan_plane(nB, nE, K, x.C, dev.cus);
-    if (!p.ok || p.NW != kXstNW || p.R != kXstR || p.P != kXstP) return false;
+    if (!p.ok || !pick_kernel(p.NW, p.R, p.P, x.C, x.lamb1, x.fast, true)) return false;
     if ((long long)x.D * x.H * x.W * 4 >= 0xFFFFFFF0ll) return false;
     return slice_offsets_fit(x, 1);
   }
@@ -1792,7 +1794,7 @@ int fg_tuning_set(const fg_tuning *t) {
   if (t->engine < 0 || t->engine > 1) return fail(FG_EINVAL, "engine must be 0 or 1");
   if (t->halo < 0 || t->halo > kMaxK) return fail(FG_EINVAL, "halo must be 0 (automatic) or in [1, 16]");
   if (t->max_groups < 0) return fail(FG_EINVAL, "max_groups must be >= 0");
-  if (t->shape < 0 || t->shape >= kNumShapes) return fail(FG_EINVAL, "shape must be in [0, 4]");
+  if (t->shape < 0 || t->shape >= kNumShapes) return fail(FG_EINVAL, "shape must be in [0, 5]");
   g_tune = *t;
   g_shape = t->shape == 0 ? kAutoShape : t->shape;
   return FG_OK;

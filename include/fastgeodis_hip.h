@@ -139,7 +139,7 @@ typedef struct fg_tuning {
   int halo;
   int max_groups; /* cap on persistent workgroups (0 = occupancy limit) */
   int shape;      /* 3D plane workgroup shape (waves x rows / slab ring): 0 auto
-                     (= 4), 1 = 4x8/2, 2 = 8x4/3, 3 = 16x2/3, 4 = 8x4/5 */
+                     (= 5), 1 = 4x8/2, 2 = 8x4/3, 3 = 16x2/3, 4 = 8x4/5, 5 = 16x2/5 */
 } fg_tuning;
 int fg_tuning_set(const fg_tuning *t);
 int fg_tuning_get(fg_tuning *t);

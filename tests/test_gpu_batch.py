@@ -97,7 +97,7 @@ def test_batch_tuning_invariance(device):
     try:
         FG.set_tuning(1, 4, 0, 0)
         ref = FG.geodesic_batch(insts)
-        for e, k, s in [(1, 2, 1), (1, 6, 3), (0, 4, 0), (1, 3, 2), (1, 5, 4), (1, 0, 4), (1, 0, 0)]:
+        for e, k, s in [(1, 2, 1), (1, 6, 3), (0, 4, 0), (1, 3, 2), (1, 5, 4), (1, 0, 4), (1, 0, 5), (1, 0, 0)]:
             FG.set_tuning(e, k, 0, s)
             outs = FG.geodesic_batch(insts)
             for a, b in zip(outs, ref):
