@@ -59,7 +59,7 @@ constexpr float kInf = __builtin_huge_valf();
 constexpr int kMaxK = 16;
 // s_sleep units (64 clocks each) between publishing a halo and first polling
 // the neighbours' publications.
-constexpr int kHaloPollDelay = 16;
+constexpr int kHaloPollDelay = 24;
 
 // ---------------------------------------------------------------------------
 // Descriptors
@@ -95,17 +95,22 @@ struct XposeDesc {  // out[c][w][h][d] = in[c][d][h][w]
 // Device helpers
 // ---------------------------------------------------------------------------
 
-// Lane l receives lane l-1 (J == 0) or lane l+1 (J == 2); the edge lane
-// receives 0.  TAG only makes each call site distinct: with bound_ctrl set
-// the "old" operand is never used, and distinct call sites keep the compiler
-// from merging shifts of one value, so every shift folds into the single
-// VALU instruction that consumes it (v_add_f32_dpp / v_sub_f32_dpp).
-template <int J, int TAG>
+// Lane l receives lane l-1 (J == 0) or lane l+1 (J == 2).  The wave's edge
+// lane receives 0 (wave shift) or the opposite edge lane's value (ALT: wave
+// rotate); that lane is always a halo lane, whose value stays outside the
+// part of the window that is exact, or a pad lane, forced to +inf each step,
+// so both forms give the same results.  A shift that several candidates use
+// is computed once by the compiler and cannot fold into its users; shifting
+// one front row for two new rows with the two different forms keeps each
+// shift single-use, so it folds into the VALU instruction that consumes it
+// (v_add_f32_dpp / v_sub_f32_dpp).  TAG keeps other call sites distinct.
+template <int J, int TAG, bool ALT = false>
 __device__ __forceinline__ float shiftE(float x) {
   if constexpr (J == 1) return x;
-  else
-    return __int_as_float(__builtin_amdgcn_update_dpp(TAG, __float_as_int(x), J == 0 ? 0x138 : 0x130, 0xF,
-                                                      0xF, true));
+  else {
+    constexpr int ctrl = (J == 0 ? 0x138 : 0x130) | (ALT ? 0x4 : 0x0);  // wave_shr/ror:1, wave_shl/rol:1
+    return __int_as_float(__builtin_amdgcn_update_dpp(TAG, __float_as_int(x), ctrl, 0xF, 0xF, true));
+  }
 }
 
 // Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
@@ -127,7 +132,7 @@ __device__ __forceinline__ float absdiff_sum(float acc, const float (&Iq)[CT], c
   if constexpr (C0 >= CT) {
     return acc;
   } else {
-    const float q = shiftE<J, 1 + C0 + I * 16 + J * 4>(Iq[C0]);
+    const float q = shiftE<J, 1 + C0 + I * 16 + J * 4, (I & 1) != 0>(Iq[C0]);
     return absdiff_sum<I, J, C0 + 1, CT>(acc + fabsf(q - In[C0]), Iq, In);
   }
 }
@@ -141,8 +146,9 @@ __device__ __forceinline__ float candidate(float dq, const float (&Iq)[CT], cons
                                            float lg) {
   // |q - p| == |p - q| exactly; the shifted operand goes first so that the
   // shift folds into v_sub_f32_dpp without commuting
-  const float l = absdiff_sum<I, J, 1, CT>(fabsf(shiftE<J, 1 + I * 16 + J * 4>(Iq[0]) - In[0]), Iq, In);
-  const float ds = shiftE<J, 64 + I * 4 + J>(dq);
+  const float l =
+      absdiff_sum<I, J, 1, CT>(fabsf(shiftE<J, 1 + I * 16 + J * 4, (I & 1) != 0>(Iq[0]) - In[0]), Iq, In);
+  const float ds = shiftE<J, 64 + I * 4 + J, (I & 1) != 0>(dq);
   if constexpr (LAMB1) {
     // l_grad == 1 and l_eucl*LD == +0: fl(1*l) == l and dropping "+ 0" is exact here
     return ds + l;
@@ -400,7 +406,7 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     }
     // Neighbours publish at about the same moment; polling right away mostly
     // finds their values not yet visible and costs a second round trip.
-    // Waiting ~1000 clocks first was measured fastest (16..24 x 64 clocks).
+    // Waiting ~1500 clocks first was measured fastest (16..24 x 64 clocks).
     __builtin_amdgcn_s_sleep(kHaloPollDelay);
     // Only lanes that need a halo value load (measured faster than letting
     // every lane load some in-bounds line and selecting afterwards).
