@@ -171,7 +171,8 @@ def main():
     traffic = read_traffic(os.path.join(ROOT, "profiles", "traffic_sweep_512.md"))
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "sweep_tiled_kernel<8 waves, 4 rows, 5-slice ring, C=1, lamb=1> (plain and transposed-output forms)", "launches": launches,
+                "kernel": "sweep_tiled_kernel<16 waves, 2 rows, 5-slice ring, C=1, lamb=1> (plain and transposed-output forms)",
+                "launches": launches,
                 "mean_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": bytes_per_launch}
     whole_gbs = updates * 12 / (ms_per_step * 1e-3) / 1e9
 
