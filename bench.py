@@ -98,6 +98,37 @@ def cpu_baseline(img_gpu, mask_gpu, iters: int):
             "sample": sample + f" ({dt:.2f} s on {cpu_model()})"}
 
 
+def c1_cpu_and_gpu():
+    """The small 2D call (512^2, C=1, lamb 1, iter 2) on the reference CPU path
+    (or the oracle port) and on the GPU: one warm-up, then the median of 10
+    CPU calls and of 50 GPU calls."""
+    import numpy as np
+
+    sys.path.insert(0, ROOT)
+    import oracle
+    from fastgeodis_amd import synthetic, workloads
+
+    img, mask = synthetic.config1("cpu")
+    ref = oracle.reference_module()
+    if ref is not None:
+        kind, fn = "reference", lambda: ref.generalised_geodesic2d(img, mask, 1e10, 1.0, 0.0, 2)
+    else:
+        kind, fn = "port", lambda: oracle.py_geodesic2d(img.numpy(), mask.numpy(), 1e10, 1.0, 2)
+    fn()
+    times = []
+    for _ in range(10):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    t_cpu = float(np.median(times))
+    gpu = workloads.run_c1()
+    n_upd = 512 * 512 * 4 * 2
+    return {"workload": "1x1x512x512 blob image, 4 seeds, lamb 1, iter 2",
+            "cpu_ms": round(t_cpu * 1e3, 3), "cpu_vups": n_upd / t_cpu, "cpu_kind": kind,
+            "cpu_threads": int(os.environ.get("OMP_NUM_THREADS", str(os.cpu_count() or 1))),
+            "gpu_median_ms": round(gpu["median_ms"], 4), "gpu_vups": gpu["vups"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,6 +145,8 @@ def main():
     ap.add_argument("--calibration-copy-mib", type=int, default=0,
                     help="after the timed region, run one plain device copy of this many MiB "
                          "(reference point for rocprofv3 byte counters)")
+    ap.add_argument("--no-copy-rate", action="store_true",
+                    help="skip the measured device-copy bandwidth")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -186,11 +219,35 @@ def main():
         log(f"[bench] calibration copy of {args.calibration_copy_mib} MiB done")
         del src, dst
 
-    cpu = None
+    # measured device-to-device copy rate beside the nominal 8 TB/s: 1 GiB
+    # read + 1 GiB written per copy, 10 copies timed with events on the stream
+    copy_res = None
+    if not args.no_copy_rate:
+        n = (1 << 30) // 4
+        src = torch.ones(n, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(10):
+            dst.copy_(src)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / 10
+        copy_res = {"bytes_per_copy": 2 * n * 4, "ms": round(ms, 4),
+                    "GBps": round(2 * n * 4 / (ms * 1e-3) / 1e9, 1),
+                    "what": "torch copy_ of 1 GiB float32 (read + write bytes / time)"}
+        log(f"[bench] device copy {copy_res['GBps']} GB/s")
+        del src, dst
+
+    cpu = c1 = None
     if not args.no_cpu_baseline:
         log("[bench] timing the CPU baseline sample")
         cpu = cpu_baseline(img, mask, args.iters)
         log(f"[bench] cpu baseline {cpu['value'] / 1e9:.3f} Gvu/s ({cpu['kind']})")
+        c1 = c1_cpu_and_gpu()
+        log(f"[bench] 512^2 2D call: cpu {c1['cpu_ms']} ms, gpu {c1['gpu_median_ms']} ms")
 
     serving_res = None
     if not args.no_serving:
@@ -232,7 +289,9 @@ def main():
                    "halo": tuning["halo"], "tile_shape": tuning["shape"]},
         "whole_call_alg_GBps": round(whole_gbs, 1),
         "roofline": roofline,
+        "hbm_copy": copy_res,
         "cpu_baseline": cpu,
+        "small_2d_call": c1,
         "click_latency": serving_res,
         "batched_vs_per_call": batch_res,
     }

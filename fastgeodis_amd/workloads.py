@@ -1,5 +1,7 @@
 """Throughput and latency workloads beside the 512^3 headline.
 
+  c1  one 512^2 image (C=1), generalised_geodesic2d lamb 1, iter 2: the small
+      2D call (median of repeated calls)
   c2  plain 3D, 1x1x128x256x256, spacing [2.5, 1, 1], unsigned and signed,
       lamb in {0, 0.5, 1}, iter 4
   c3  32 RGB 1024^2 images: generalised_geodesic2d (lamb 0.5, iter 2) and
@@ -45,6 +47,21 @@ def _timed(fn, reps: int):
 
 def _same(a: torch.Tensor, b: torch.Tensor) -> bool:
     return a.shape == b.shape and torch.equal(a.contiguous().view(torch.int32), b.contiguous().view(torch.int32))
+
+
+def run_c1(device="cuda", reps: int = 50):
+    img, mask = synthetic.config1(device)
+    generalised_geodesic2d(img, mask, 1e10, 1.0, 2)
+    times = []
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        generalised_geodesic2d(img, mask, 1e10, 1.0, 2)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    t = float(np.median(times))
+    return {"workload": "c1 1x1x512x512 lamb 1 iter 2", "median_ms": t * 1e3,
+            "min_ms": float(np.min(times)) * 1e3, "vups": _updates(img, 2) / t}
 
 
 def run_c2(device="cuda", reps: int = 5):
@@ -120,7 +137,7 @@ def run_c5(rates=(50, 200, 500, 1000, 2000), requests: int = 1000, sessions: int
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("which", nargs="+", choices=["c2", "c3", "c4", "c5"])
+    ap.add_argument("which", nargs="+", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--rates", type=float, nargs="+", default=[50, 200, 500, 1000, 2000])
     ap.add_argument("--requests", type=int, default=1000)
     args = ap.parse_args(argv)
@@ -132,7 +149,7 @@ def main(argv=None):
         if w == "c5":
             res = run_c5(args.rates, args.requests)
         else:
-            res = {"c2": run_c2, "c3": run_c3, "c4": run_c4}[w]()
+            res = {"c1": run_c1, "c2": run_c2, "c3": run_c3, "c4": run_c4}[w]()
         print(json.dumps(res), flush=True)
     return 0
 
