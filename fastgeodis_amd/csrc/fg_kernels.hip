@@ -1059,6 +1059,7 @@ int coresident_groups(const void *kern, int threads, const Device &dev, long lon
 struct SweepJob {
   Xform *x;
   int pass;
+  int K;  // halo depth of this plane's plan (> 0)
 };
 
 struct Runner {
@@ -1149,7 +1150,7 @@ struct Runner {
       SweepDesc s;
       fill_desc(*j.x, j.pass, dir, s, xs);
       alg += (double)(s.nA - 1) * s.nB * s.nE * (4.0 * s.C + 8.0);
-      Plan p = plan_plane(s.nB, s.nE, K, s.C, dev.cus);
+      Plan p = plan_fixed(s.nB, s.nE, j.K, s.C);
       if (!p.ok || p.NW != NW || p.R != R || p.P != P || p.K % P != 0 || (kk >= 0 && kk != p.K))
         return fail(FG_EINVAL, "internal: sweep plan does not match the selected kernel");
       s.nTB = p.nTB; s.nTE = p.nTE; s.TB = p.TB; s.TE = p.TE;
@@ -1239,30 +1240,93 @@ struct Runner {
     else { D = x.D; H = x.H; W = x.W; }
   }
 
+  // Whether a transform's sweeps of this pass run on the tiled engine.
+  bool tiled_eligible(const Xform &x, int pass) {
+    int nA, nB, nE;
+    pass_plane(x, pass, nA, nB, nE);
+    if (nA <= 1 || engine == 0 || x.C > 4) return false;
+    return plan_plane(nB, nE, K, x.C, dev.cus).ok && slice_offsets_fit(x, pass);
+  }
+
+  // Halo depth of each transform's plane in this pass.  Requested depths
+  // apply as they are.  Automatic: a 2D row (one-wave tiles, bound by the
+  // exchange, not by arithmetic) takes the deepest halo whose tiles fit one
+  // per CU, as does a lone 3D plane (idle CUs cost nothing, fewer exchanges
+  // shorten the sweep).  3D planes that share the pass are planned together:
+  // the deepest halo at which all their tiles still fit one per CU, and the
+  // shallowest one when none does.  Their launches then run back to back,
+  // and a deeper halo would only add window arithmetic (a 32 x 64 window
+  // computes 1.7 x its own tile at K = 5 and 3.9 x at K = 10).
+  std::vector<int> pass_halos(const std::vector<Xform *> &active, int pass) {
+    std::vector<int> ks(active.size(), 0);
+    std::vector<int> shared;  // indices of the automatic 3D planes
+    for (size_t i = 0; i < active.size(); ++i) {
+      int nA, nB, nE;
+      pass_plane(*active[i], pass, nA, nB, nE);
+      if (!tiled_eligible(*active[i], pass)) continue;
+      ks[i] = plan_plane(nB, nE, K, active[i]->C, dev.cus).K;
+      if (K == 0 && nB > 1) shared.push_back((int)i);
+    }
+    if (shared.size() < 2) return ks;
+    // one depth per slab-ring length (channel classes use different rings)
+    std::vector<int> rings;
+    for (int i : shared) {
+      int nA, nB, nE;
+      pass_plane(*active[i], pass, nA, nB, nE);
+      const int P = plan_fixed(nB, nE, 1, active[i]->C).P;
+      if (std::find(rings.begin(), rings.end(), P) == rings.end()) rings.push_back(P);
+    }
+    for (int P : rings) {
+      int best = P;
+      for (int k = P; k <= kMaxK; k += P) {
+        long long tiles = 0;
+        bool ok = true;
+        for (int i : shared) {
+          int nA, nB, nE;
+          pass_plane(*active[i], pass, nA, nB, nE);
+          const Plan q = plan_fixed(nB, nE, k, active[i]->C);
+          if (plan_fixed(nB, nE, 1, active[i]->C).P != P) continue;
+          if (!q.ok) { ok = false; break; }
+          tiles += (long long)q.nTB * q.nTE;
+        }
+        if (!ok || tiles > dev.cus) break;
+        best = k;
+      }
+      for (int i : shared) {
+        int nA, nB, nE;
+        pass_plane(*active[i], pass, nA, nB, nE);
+        if (plan_fixed(nB, nE, 1, active[i]->C).P == P) ks[i] = best;
+      }
+    }
+    return ks;
+  }
+
   // Record one pass (forward then backward) for the active transforms.
   int sweep_pass(std::vector<Xform *> &active, int pass) {
+    const std::vector<int> ks = pass_halos(active, pass);
     for (int dir : {+1, -1}) {
       struct G { int NW, R, P, K, C; bool l1, fa, xs; std::vector<SweepJob> jobs; };
       std::vector<G> groups;
-      for (auto *x : active) {
+      for (size_t i = 0; i < active.size(); ++i) {
+        Xform *x = active[i];
         int nA, nB, nE;
         pass_plane(*x, pass, nA, nB, nE);
         if (nA <= 1) continue;
-        Plan p = plan_plane(nB, nE, K, x->C, dev.cus);
-        if (engine == 0 || !p.ok || x->C > 4 || !slice_offsets_fit(*x, pass)) {
+        if (ks[i] == 0) {
           record_per_slice(*x, pass, dir);
           continue;
         }
+        const Plan p = plan_fixed(nB, nE, ks[i], x->C);
         const bool xs = pass == 1 && dir < 0 && x->xst;
         bool found = false;
         for (auto &g : groups)
           if (g.NW == p.NW && g.R == p.R && g.P == p.P && g.K == p.K && g.C == x->C && g.l1 == x->lamb1 &&
               g.fa == x->fast && g.xs == xs) {
-            g.jobs.push_back({x, pass});
+            g.jobs.push_back({x, pass, p.K});
             found = true;
             break;
           }
-        if (!found) groups.push_back({p.NW, p.R, p.P, p.K, x->C, x->lamb1, x->fast, xs, {{x, pass}}});
+        if (!found) groups.push_back({p.NW, p.R, p.P, p.K, x->C, x->lamb1, x->fast, xs, {{x, pass, p.K}}});
       }
       for (auto &g : groups) {
         int rc = record_group(g.jobs, dir, g.NW, g.R, g.P, g.C, g.l1, g.fa, g.xs);
@@ -1294,7 +1358,7 @@ struct Runner {
     for (auto &j : jobs) {
       int nA, nB, nE;
       pass_plane(*j.x, j.pass, nA, nB, nE);
-      Plan p = plan_plane(nB, nE, K, j.x->C, dev.cus);
+      Plan p = plan_fixed(nB, nE, j.K, j.x->C);
       const long long nt = (long long)p.nTB * p.nTE;
       if (nt > cap) {  // one plane alone cannot be co-resident: per-slice engine
         record_per_slice(*j.x, j.pass, dir);
