@@ -48,6 +48,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -389,6 +392,7 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   // between the halo loads and the wait for them, so that wait does not also
   // wait for the new prefetch; pending rows are polled together, so one
   // memory round trip serves all of them.
+  bool wd_tripped = false;
   auto exchange = [&](int blk, Slab<R, CT> &pf_slab, int pf_step) {
     const unsigned tag = tag_base + (unsigned)blk;
     const long long hslot = (long long)(blk & 1) * tile_slots;
@@ -428,7 +432,10 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
       if (need[r]) x[r] = __hip_atomic_load(src[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     load_slab(pf_slab, min(max(a_first + pf_step * dir, 0), nA - 1));
-    int spins = 0;
+    // after an expiry this wave waits no more: the sweep then finishes
+    // (with wrong values, reported through the status word) instead of
+    // spending the full limit again at every later exchange
+    int spins = wd_tripped ? spin_limit : 0;
     while (true) {
       bool waiting = false;
 #pragma unroll
@@ -436,6 +443,7 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
       if (!waiting) break;
       if (++spins > spin_limit) {
         __hip_atomic_store(status, FG_EWATCHDOG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        wd_tripped = true;
         break;
       }
 #pragma unroll
@@ -1033,13 +1041,59 @@ struct Device {
 int query_device(Device &dev) {
   int id = 0;
   FG_HIP(hipGetDevice(&id));
-  int cus = 0, coop = 0;
+  int cus = 0;
   FG_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, id));
-  FG_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, id));
-  if (!coop) return fail(FG_EHIP, "device does not support cooperative launches");
   dev.id = id;
   dev.cus = cus;
   return FG_OK;
+}
+
+// The tiled sweep needs all workgroups of a launch resident at once.  Its
+// grid never exceeds the occupancy limit (coresident_groups), so on its own
+// every workgroup starts; but two such launches running together (calls on
+// different streams) could each hold part of the device and wait on
+// workgroups that cannot start.  Calls that launch tiled sweeps are therefore
+// ordered per device: the host lock covers a whole replay, and a call on a
+// different stream than the previous one first waits for that call's end.
+// (Plain launches: the cooperative launch API adds ~15 us per launch.)
+struct DeviceOrder {
+  std::mutex m;
+  bool has = false;
+  hipStream_t last{};
+  hipEvent_t done{};
+};
+
+// Holds a device's order for one replay; on every way out (errors included)
+// marks the end of what the replay enqueued on its stream.
+struct OrderedReplay {
+  DeviceOrder *ord = nullptr;
+  hipStream_t stream{};
+  std::unique_lock<std::mutex> lock;
+  int begin(DeviceOrder &o, hipStream_t s) {
+    lock = std::unique_lock<std::mutex>(o.m);
+    if (!o.has) {
+      FG_HIP(hipEventCreateWithFlags(&o.done, hipEventDisableTiming));
+      o.has = true;
+    } else if (o.last != s) {
+      FG_HIP(hipStreamWaitEvent(s, o.done, 0));
+    }
+    o.last = s;
+    ord = &o;
+    stream = s;
+    return FG_OK;
+  }
+  ~OrderedReplay() {
+    if (ord) (void)hipEventRecord(ord->done, stream);
+  }
+};
+
+DeviceOrder &device_order(int id) {
+  static std::mutex reg_m;
+  static std::map<int, std::unique_ptr<DeviceOrder>> reg;
+  std::lock_guard<std::mutex> g(reg_m);
+  auto &slot = reg[id];
+  if (!slot) slot.reset(new DeviceOrder());
+  return *slot;
 }
 
 // Occupancy-limited number of co-resident workgroups per kernel (cached per
@@ -1412,9 +1466,16 @@ struct Runner {
         for (int i = 0; i < c.ndesc; ++i)
           if (ds[i].kind == EW_FILL_U64) ds[i].u64 = halo;
       }
+    bool tiled = false;
+    for (auto &c : rec.cmds) tiled = tiled || c.type == Cmd::TILED;
+    OrderedReplay order;
+    if (tiled) {
+      const int rc = order.begin(device_order(dev.id), stream);
+      if (rc) return rc;
+    }
     if (!rec.blob.empty())
       FG_HIP(hipMemcpyAsync(blob_dev, rec.blob.data(), rec.blob.size(), hipMemcpyHostToDevice, stream));
-    const int spin = 1 << 22;
+    const int spin = 1 << 21;
     for (auto &c : rec.cmds) {
       switch (c.type) {
         case Cmd::STATUS:
@@ -1431,7 +1492,7 @@ struct Runner {
             FG_HIP(hipEventCreate(&pr.stop));
             FG_HIP(hipEventRecord(pr.start, stream));
           }
-          FG_HIP(hipLaunchCooperativeKernel(c.kern, dim3(c.grid), dim3(c.block), args, 0, stream));
+          FG_HIP(hipLaunchKernel(c.kern, dim3(c.grid), dim3(c.block), args, 0, stream));
           if (g_prof_on) {
             FG_HIP(hipEventRecord(pr.stop, stream));
             pr.bytes = c.alg_bytes;
