@@ -59,7 +59,10 @@
 namespace {
 
 constexpr float kInf = __builtin_huge_valf();
-constexpr int kMaxK = 16;
+// Deepest halo: 2D rows use up to 24 of a 64-lane window (one-wave tiles are
+// bound by the exchange, and a lone image leaves most CUs idle); 3D windows
+// of 32 rows stop at 15 by themselves.
+constexpr int kMaxK = 24;
 // s_sleep units (64 clocks each) between publishing a halo and first polling
 // the neighbours' publications.
 constexpr int kHaloPollDelay = 24;
@@ -865,7 +868,7 @@ struct Shape {
 };
 // Workgroup shapes with compiled kernels: index 0 serves 2D row sweeps
 // (one row); the others tile 3D planes (selected by fg_tuning.shape).
-constexpr Shape kShapes[] = {{1, 1, 4, 4}, {4, 8, 2, 2}, {8, 4, 3, 2}, {16, 2, 3, 3}, {8, 4, 5, 2}, {16, 2, 5, 3}};
+constexpr Shape kShapes[] = {{1, 1, 8, 8}, {4, 8, 2, 2}, {8, 4, 3, 2}, {16, 2, 3, 3}, {8, 4, 5, 2}, {16, 2, 5, 3}};
 constexpr int kNumShapes = 6;
 constexpr int kAutoShape = 5;  // fastest on 512^2 planes: 16 x 2 with the deepest ring that keeps K = 5
 thread_local int g_shape = kAutoShape;
@@ -974,7 +977,7 @@ const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast, bool
   }
 #define FG_S(nw, r, p, clo, chi) \
   if (NW == nw && R == r && P == p && C >= clo && C <= chi) return pick_kernel_nr<nw, r, p, clo, chi>(C, lamb1, fast);
-  FG_S(1, 1, 4, 1, 4)
+  FG_S(1, 1, 8, 1, 4)
   FG_S(4, 8, 2, 1, 4)
   FG_S(8, 4, 3, 1, 2)
   FG_S(8, 4, 5, 1, 2)
@@ -1303,56 +1306,62 @@ struct Runner {
   }
 
   // Halo depth of each transform's plane in this pass.  Requested depths
-  // apply as they are.  Automatic: a 2D row (one-wave tiles, bound by the
-  // exchange, not by arithmetic) takes the deepest halo whose tiles fit one
-  // per CU, as does a lone 3D plane (idle CUs cost nothing, fewer exchanges
-  // shorten the sweep).  3D planes that share the pass are planned together:
-  // the deepest halo at which all their tiles still fit one per CU, and the
-  // shallowest one when none does.  Their launches then run back to back,
-  // and a deeper halo would only add window arithmetic (a 32 x 64 window
-  // computes 1.7 x its own tile at K = 5 and 3.9 x at K = 10).
+  // apply as they are.  Automatic: a lone plane takes the deepest halo whose
+  // tiles fit one per CU (idle CUs cost nothing, and fewer exchanges shorten
+  // the sweep).  Planes that share the pass (batches, the halves of a signed
+  // or GSF call) are planned together: the deepest halo at which all their
+  // tiles still fit the device, and the shallowest one when none does, as a
+  // deeper halo then only adds window arithmetic (a 32 x 64 window computes
+  // 1.7 x its own tile at K = 5 and 3.9 x at K = 10; a 64-lane row 1.3 x at
+  // K = 8 and 4 x at K = 24).  3D tiles fill a CU each; 2D rows are one-wave
+  // tiles bound by latency, so they fit one per SIMD.
   std::vector<int> pass_halos(const std::vector<Xform *> &active, int pass) {
     std::vector<int> ks(active.size(), 0);
-    std::vector<int> shared;  // indices of the automatic 3D planes
+    std::vector<int> rows, planes;  // automatic 2D rows and 3D planes
     for (size_t i = 0; i < active.size(); ++i) {
       int nA, nB, nE;
       pass_plane(*active[i], pass, nA, nB, nE);
       if (!tiled_eligible(*active[i], pass)) continue;
       ks[i] = plan_plane(nB, nE, K, active[i]->C, dev.cus).K;
-      if (K == 0 && nB > 1) shared.push_back((int)i);
+      if (K == 0) (nB > 1 ? planes : rows).push_back((int)i);
     }
-    if (shared.size() < 2) return ks;
-    // one depth per slab-ring length (channel classes use different rings)
-    std::vector<int> rings;
-    for (int i : shared) {
+    if (rows.size() > 1) share_halo(active, pass, rows, 4LL * dev.cus, ks);
+    if (planes.size() > 1) share_halo(active, pass, planes, dev.cus, ks);
+    return ks;
+  }
+
+  // One halo depth per slab-ring length for the planes `idx` (channel
+  // classes use different rings): the deepest whose tiles total at most
+  // `budget`, else the shallowest.
+  void share_halo(const std::vector<Xform *> &active, int pass, const std::vector<int> &idx, long long budget,
+                  std::vector<int> &ks) {
+    auto ring = [&](int i) {
       int nA, nB, nE;
       pass_plane(*active[i], pass, nA, nB, nE);
-      const int P = plan_fixed(nB, nE, 1, active[i]->C).P;
-      if (std::find(rings.begin(), rings.end(), P) == rings.end()) rings.push_back(P);
-    }
+      return plan_fixed(nB, nE, 1, active[i]->C).P;
+    };
+    std::vector<int> rings;
+    for (int i : idx)
+      if (std::find(rings.begin(), rings.end(), ring(i)) == rings.end()) rings.push_back(ring(i));
     for (int P : rings) {
       int best = P;
       for (int k = P; k <= kMaxK; k += P) {
         long long tiles = 0;
         bool ok = true;
-        for (int i : shared) {
+        for (int i : idx) {
+          if (ring(i) != P) continue;
           int nA, nB, nE;
           pass_plane(*active[i], pass, nA, nB, nE);
           const Plan q = plan_fixed(nB, nE, k, active[i]->C);
-          if (plan_fixed(nB, nE, 1, active[i]->C).P != P) continue;
           if (!q.ok) { ok = false; break; }
           tiles += (long long)q.nTB * q.nTE;
         }
-        if (!ok || tiles > dev.cus) break;
+        if (!ok || tiles > budget) break;
         best = k;
       }
-      for (int i : shared) {
-        int nA, nB, nE;
-        pass_plane(*active[i], pass, nA, nB, nE);
-        if (plan_fixed(nB, nE, 1, active[i]->C).P == P) ks[i] = best;
-      }
+      for (int i : idx)
+        if (ring(i) == P) ks[i] = best;
     }
-    return ks;
   }
 
   // Record one pass (forward then backward) for the active transforms.
@@ -1923,7 +1932,7 @@ int fg_geodesic_batch_f32(const fg_instance *inst, int n, void *workspace, size_
 int fg_tuning_set(const fg_tuning *t) {
   if (!t) return fail(FG_EINVAL, "null tuning");
   if (t->engine < 0 || t->engine > 1) return fail(FG_EINVAL, "engine must be 0 or 1");
-  if (t->halo < 0 || t->halo > kMaxK) return fail(FG_EINVAL, "halo must be 0 (automatic) or in [1, 16]");
+  if (t->halo < 0 || t->halo > kMaxK) return fail(FG_EINVAL, "halo must be 0 (automatic) or in [1, 24]");
   if (t->max_groups < 0) return fail(FG_EINVAL, "max_groups must be >= 0");
   if (t->shape < 0 || t->shape >= kNumShapes) return fail(FG_EINVAL, "shape must be in [0, 5]");
   g_tune = *t;

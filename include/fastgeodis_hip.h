@@ -6,9 +6,12 @@
  * current HIP device; `stream` is a hipStream_t passed as void* (NULL = the
  * null stream).  Calls are asynchronous on `stream` and allocate nothing:
  * scratch memory comes from the caller's `workspace` (query its size with
- * fg_workspace_bytes).  The library keeps no mutable global state, so calls
- * from several host threads on different streams may overlap, each with its
- * own workspace.
+ * fg_workspace_bytes).  Calls may come from several host threads on different
+ * streams, each with its own workspace.  The tiled sweep kernels need all of
+ * a launch's workgroups resident at once, so the library orders the work of
+ * such calls per device: a call on a different stream than the previous one
+ * waits (on the device) for that call's end.  The only shared state is that
+ * order and read-only caches.
  *
  * Layouts (batch 1, C-contiguous float32):
  *   image (C, [D,] H, W), mask (Cm, [D,] H, W), out (Cm, [D,] H, W).
@@ -130,10 +133,12 @@ int fg_geodesic_batch_f32(const fg_instance *inst, int n, void *workspace,
 
 /* Tile/launch parameters of the persistent sweep engine.  engine 0 is the
  * one-launch-per-slice engine, 1 the persistent tiled engine (default).
- * halo is the trapezoid depth K (slices per neighbour exchange), rounded up
- * to a multiple of the kernel's slab ring; 0 (default) picks, per sweep plane,
- * the deepest halo whose tiles fit one per compute unit.  Values are per host
- * thread; fg_tuning_get reads them back. */
+ * halo is the trapezoid depth K (slices per neighbour exchange, 1..24),
+ * rounded up to a multiple of the kernel's slab ring; 0 (default) picks the
+ * deepest halo whose tiles fit one per compute unit: per 2D row and per lone
+ * 3D plane, and over all 3D planes that share a pass (batches, the halves of
+ * a signed or GSF call).  Values are per host thread; fg_tuning_get reads
+ * them back. */
 typedef struct fg_tuning {
   int engine;
   int halo;
