@@ -1,8 +1,8 @@
 """Empirical tuner for the persistent sweep engine.
 
-The engine's own defaults are analytic: the 8 x 4 workgroup with a 5-slice
-prefetch ring and, per plane, the deepest halo whose tiles fit one per
-compute unit.  For other problem sizes a different workgroup shape or halo
+The engine's own defaults are analytic: the 16 x 2 workgroup with a 5-slice
+prefetch ring and the deepest halo whose tiles fit the device (per lone
+plane, or over the planes that share a pass).  For other problem sizes a different workgroup shape or halo
 depth can be faster; this module measures that directly: it builds a
 synthetic blob-field input of the requested size on the device, times the
 plain transform under each candidate ``(engine, halo, shape)`` (see

@@ -21,7 +21,7 @@
 // E inner), so the exact-order path visits candidates in the reference order.
 //
 // Engines.
-//   tiled (default): one persistent, cooperatively launched kernel per sweep.
+//   tiled (default): one persistent kernel per sweep (all workgroups resident).
 //     A workgroup owns a tile of the slice plane and walks along A.  Its
 //     front (the previous slice) lives in VGPRs: lane = E position, register
 //     = B row; E neighbours arrive through DPP wave shifts folded into the
