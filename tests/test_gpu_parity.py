@@ -113,6 +113,15 @@ RANDOM_3D = [
     ((5, 130, 33), [3.0, 0.5, 1.0], 5, 0.6, 2),      # C=5: per-slice fallback
     ((40, 40, 40), [1.0, 1.0, 1.0], 1, 1.25, 2),     # lamb > 1: exact-order path
     ((31, 47, 71), [1.0, 1.0, 1.0], 1, -0.5, 2),     # negative lamb: exact-order path
+    # degenerate sides: a pass with one slice does nothing, planes of one or
+    # two rows/lanes, a single voxel
+    ((1, 50, 70), [1.0, 1.0, 1.0], 1, 1.0, 2),
+    ((30, 1, 90), [2.0, 1.0, 1.0], 2, 0.5, 2),
+    ((25, 40, 1), [1.0, 1.0, 1.0], 1, 0.0, 3),
+    ((2, 2, 2), [1.0, 1.0, 1.0], 1, 1.0, 2),
+    ((1, 1, 1), [1.0, 1.0, 1.0], 1, 0.5, 2),
+    ((3, 200, 2), [1.0, 0.5, 2.0], 1, 0.7, 2),
+    ((64, 2, 300), [1.5, 1.0, 1.0], 3, 1.0, 2),
 ]
 
 
