@@ -687,51 +687,46 @@ struct EwDesc {
   float v, theta;
   unsigned long long *u64;
   long long nu64;
-  long long elem0;  // first element of this descriptor in the launch
+  long long blk0;   // first block of this descriptor in the launch
   long long nelem;
 };
+
+// Elements per ew_kernel block (256 threads x 4).
+constexpr int kEwPerBlock = 1024;
 
 __device__ __forceinline__ float signed_closed(float v, float m) {
   // untransformed channels of signed: v*m - v*(1-m)
   return v * m - v * (1.0f - m);
 }
 
-__device__ __forceinline__ int ew_find(const EwDesc *__restrict__ descs, int ndesc, long long g) {
+__device__ __forceinline__ void ew_one(const EwDesc &E, long long i);
+
+// Each block handles kEwPerBlock consecutive elements of one descriptor (the
+// block-uniform search runs on scalar loads once per block), each thread 4
+// of them; the unsigned initialisation (every plain call) uses one 16-byte
+// load and store when the 4 elements are aligned.
+__global__ __launch_bounds__(256) void ew_kernel(const EwDesc *__restrict__ descs, int ndesc) {
+  const long long b = blockIdx.x;
   int lo = 0, hi = ndesc - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (descs[mid].elem0 <= g) lo = mid; else hi = mid - 1;
+    if (descs[mid].blk0 <= b) lo = mid; else hi = mid - 1;
   }
-  return lo;
-}
-
-__device__ __forceinline__ void ew_one(const EwDesc &E, long long i);
-
-// Each thread handles 4 consecutive elements; the unsigned initialisation
-// (every plain call) uses one 16-byte load and store when the 4 elements
-// belong to one descriptor and are aligned.
-__global__ __launch_bounds__(256) void ew_kernel(const EwDesc *__restrict__ descs, int ndesc,
-                                                  long long total) {
-  for (long long g0 = ((long long)blockIdx.x * blockDim.x + threadIdx.x) * 4; g0 < total;
-       g0 += (long long)gridDim.x * blockDim.x * 4) {
-    const EwDesc &E = descs[ew_find(descs, ndesc, g0)];
-    const long long i0 = g0 - E.elem0;
-    if (E.kind == EW_INIT_UNSIGNED && i0 + 4 <= E.nelem &&
-        (((uintptr_t)(E.out + i0) | (uintptr_t)(E.mask + i0)) & 15) == 0) {
-      const float4 m = *reinterpret_cast<const float4 *>(E.mask + i0);
-      float4 o;
-      o.x = E.v * m.x;
-      o.y = E.v * m.y;
-      o.z = E.v * m.z;
-      o.w = E.v * m.w;
-      *reinterpret_cast<float4 *>(E.out + i0) = o;
-      continue;
-    }
-    for (int j = 0; j < 4 && g0 + j < total; ++j) {
-      const EwDesc &F = descs[ew_find(descs, ndesc, g0 + j)];
-      ew_one(F, g0 + j - F.elem0);
-    }
+  const EwDesc &E = descs[lo];
+  const long long i0 = (b - E.blk0) * kEwPerBlock + (long long)threadIdx.x * 4;
+  if (i0 >= E.nelem) return;
+  if (E.kind == EW_INIT_UNSIGNED && i0 + 4 <= E.nelem &&
+      (((uintptr_t)(E.out + i0) | (uintptr_t)(E.mask + i0)) & 15) == 0) {
+    const float4 m = *reinterpret_cast<const float4 *>(E.mask + i0);
+    float4 o;
+    o.x = E.v * m.x;
+    o.y = E.v * m.y;
+    o.z = E.v * m.z;
+    o.w = E.v * m.w;
+    *reinterpret_cast<float4 *>(E.out + i0) = o;
+    return;
   }
+  for (int j = 0; j < 4 && i0 + j < E.nelem; ++j) ew_one(E, i0 + j);
 }
 
 __device__ __forceinline__ void ew_one(const EwDesc &E, long long i) {
@@ -1013,8 +1008,7 @@ struct Cmd {
   int ndesc = 0;
   int K = 0;
   unsigned tag_base = 0;
-  long long total = 0;  // EW: elements
-  SliceDesc slice{};    // SLICE: descriptor by value
+  SliceDesc slice{};  // SLICE: descriptor by value
   int a = 0, aq = 0;
   double alg_bytes = 0;  // TILED: algorithmic HBM bytes of the launch
 };
@@ -1253,19 +1247,18 @@ struct Runner {
 
   void record_ew(const std::vector<EwDesc> &in) {
     std::vector<EwDesc> ds = in;
-    long long total = 0;
+    long long blocks = 0;
     for (auto &e : ds) {
-      e.elem0 = total;
-      total += e.nelem;
+      e.blk0 = blocks;
+      blocks += ceil_div(e.nelem, (long long)kEwPerBlock);
     }
-    if (total == 0) return;
+    if (blocks == 0) return;
     Cmd c;
     c.type = Cmd::EW;
     c.off = rec.put(ds.data(), ds.size() * sizeof(EwDesc));
     c.ndesc = (int)ds.size();
-    c.total = total;
     c.block = 256;
-    c.grid = (int)std::max<long long>(1, std::min<long long>(ceil_div(total, 1024), (long long)dev.cus * 16));
+    c.grid = (int)blocks;  // < 2^31: 2^41 elements would not fit any device
     rec.cmds.push_back(c);
   }
 
@@ -1519,7 +1512,7 @@ struct Runner {
         }
         case Cmd::EW: {
           const EwDesc *dp = reinterpret_cast<const EwDesc *>(blob_dev + c.off);
-          hipLaunchKernelGGL(ew_kernel, dim3(c.grid), dim3(c.block), 0, stream, dp, c.ndesc, c.total);
+          hipLaunchKernelGGL(ew_kernel, dim3(c.grid), dim3(c.block), 0, stream, dp, c.ndesc);
           break;
         }
       }
