@@ -125,3 +125,26 @@ def test_empty_instance_in_batch_and_zero_channel_image(device):
     with pytest.raises(ValueError):
         FG.generalised_geodesic2d(torch.rand(1, 0, 8, 8, device="cuda"), torch.ones(1, 1, 8, 8, device="cuda"),
                                   1e10, 1.0, 2)
+
+
+@pytest.mark.parametrize("seed", [21, 22])
+def test_shared_halo_planning_mixed_rings(device, seed):
+    """3D planes of 1-4 channels (two slab-ring lengths) and 2D rows share the
+    passes of one batch; enough of them that their tiles exceed the device, so
+    the shared halo depth differs from each plane's own: every result must
+    still equal its single call."""
+    rng = np.random.default_rng(seed)
+    insts = []
+    for k in range(14):
+        shape = tuple(int(x) for x in rng.integers(24, 120, size=3))
+        sp = [float(x) for x in rng.uniform(0.5, 3.0, size=3)]
+        insts.append(rand_instance(rng, 3, 1 + k % 4, shape, ["unsigned", "signed", "gsf"][k % 3],
+                                   float(rng.choice([0.0, 0.5, 1.0])), int(rng.integers(1, 3)), sp))
+    for k in range(12):
+        shape = (int(rng.integers(30, 700)), int(rng.integers(30, 900)))
+        insts.append(rand_instance(rng, 2, 1 + k % 3, shape, ["unsigned", "signed"][k % 2], 1.0, 2))
+    print("shared-halo batch of", len(insts), flush=True)
+    outs = FG.geodesic_batch(insts)
+    torch.cuda.synchronize()
+    for k, (inst, out) in enumerate(zip(insts, outs)):
+        assert bits_equal(out, single(inst)), f"instance {k} ({inst.op}, {tuple(inst.image.shape)}) differs"
