@@ -17,9 +17,13 @@ Prints ONE JSON line on stdout; progress goes to stderr.
              during the timed region, against MI355X's 8 TB/s HBM peak;
              traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE +
              WRITE_SIZE (profiles/traffic_sweep_512.md), or null
+  hbm_copy   measured device-to-device copy rate (1 GiB torch copy_, read +
+             write bytes / time), the practical ceiling beside the nominal peak
   cpu_baseline  the reference's own OpenMP CPU path (compiled from its
              sources into oracle/_ref) or, without it, the C restatement
              (oracle/), on the same volume and call, run once
+  small_2d_call  the 512^2 2D call (C=1, lamb 1, iter 2) on that CPU path
+             (median of 10) and on the GPU (median of 50)
   click_latency  p50/p99 of the serving loop (64 resident sessions, sides
              96-320, signed iter 2) at 50 and 200 offered clicks/s
   batched_vs_per_call  one batched call against a loop of plain calls on 256
