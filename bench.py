@@ -155,7 +155,7 @@ def main():
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build runs the transform on a single MI355X "
+        print("bench.py measures one GPU: the transform runs on a single MI355X "
               "(no multi-GPU path); rerun with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
