@@ -14,16 +14,20 @@ Prints ONE JSON line on stdout; progress goes to stderr.
   roofline   the dominant kernel (persistent tiled sweep): algorithmic bytes
              (12 B per voxel update at C=1) per launch / its mean duration,
              measured with HIP events around each launch on its own stream
-             during the timed region, against MI355X's 8 TB/s HBM peak;
+             in calls after the timed region (the timed calls carry no
+             events), against MI355X's 8 TB/s HBM peak;
              traffic = HBM bytes per launch from rocprofv3 FETCH_SIZE +
              WRITE_SIZE (profiles/traffic_sweep_512.md), or null
   hbm_copy   measured device-to-device copy rate (1 GiB torch copy_, read +
              write bytes / time), the practical ceiling beside the nominal peak
-  cpu_baseline  the reference's own OpenMP CPU path (compiled from its
-             sources into oracle/_ref) or, without it, the C restatement
-             (oracle/), on the same volume and call, run once
+  cpu_baseline  the C restatement of the reference's OpenMP CPU path
+             (oracle/fg_oracle.c, bit-exact with the reference) on the same
+             volume and call: 1 warm-up, median of 5 on all host threads, plus
+             a 1-thread rate on a 16-plane slab; host fields (nproc, affinity,
+             CPU model, OMP_NUM_THREADS)
   small_2d_call  the 512^2 2D call (C=1, lamb 1, iter 2) on that CPU path
-             (median of 10) and on the GPU (median of 50)
+             (median of 10, all threads and 1 thread) and on the GPU (median
+             of 50)
   click_latency  p50/p99 of the serving loop (64 resident sessions, sides
              96-320, signed iter 2) at 50 and 200 offered clicks/s
   batched_vs_per_call  one batched call against a loop of plain calls on 256
@@ -69,67 +73,86 @@ def read_traffic(path: str):
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(img_gpu, mask_gpu, iters: int):
-    """Times the reference CPU path (or the oracle port) on a bounded sample."""
-    import torch
+def _median_time(fn, runs: int) -> float:
+    import numpy as np
 
+    fn()  # warm-up
+    times = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times))
+
+
+def host_fields() -> dict:
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    return {"nproc": os.cpu_count(), "affinity_cores": affinity, "cpu_model": cpu_model(),
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def cpu_baseline(img_gpu, mask_gpu, iters: int, runs: int = 5, slab_1t: int = 16):
+    """The C restatement of the reference's OpenMP CPU path (oracle/fg_oracle.c,
+    layout-faithful form: per-pass contiguous copies like the reference's
+    transposes), bit-exact with the reference: 1 warm-up then the median of
+    `runs` calls of the full volume on all threads, and a 1-thread figure on a
+    `slab_1t`-deep slab of the same volume (median of 3)."""
     sys.path.insert(0, ROOT)
     import oracle
 
-    sample_d = int(os.environ.get("FG_CPU_SAMPLE_DEPTH", str(img_gpu.shape[2])))
-    img = img_gpu[:, :, :sample_d].contiguous().cpu()
-    mask = mask_gpu[:, :, :sample_d].contiguous().cpu()
-    n = img[0, 0].numel()
-    threads = int(os.environ.get("OMP_NUM_THREADS", str(os.cpu_count() or 1)))
-    ref = oracle.reference_module()
-    S = img_gpu.shape[3]
-    sample = (f"the same {sample_d}x{S}x{S} volume and call (lamb=1.0, iter={iters}), one run"
-              if sample_d == img_gpu.shape[2] else
-              f"{sample_d}x{S}x{S} slab of the same volume, lamb=1.0, iter={iters}, one run")
-    if ref is not None:
-        kind = "reference"
-        t0 = time.perf_counter()
-        ref.generalised_geodesic3d(img, mask, [1.0, 1.0, 1.0], 1e10, 1.0, 0.0, iters)
-        dt = time.perf_counter() - t0
-    else:
-        kind = "port"
-        t0 = time.perf_counter()
-        oracle.py_geodesic3d(img.numpy(), mask.numpy(), [1.0, 1.0, 1.0], 1e10, 1.0, iters)
-        dt = time.perf_counter() - t0
-        threads = oracle.num_threads()
-    rate = n * 6 * iters / dt
-    return {"value": rate, "unit": "voxel-updates/s", "cores": threads, "kind": kind,
-            "sample": sample + f" ({dt:.2f} s on {cpu_model()})"}
+    img = img_gpu.cpu().numpy()
+    mask = mask_gpu.cpu().numpy()
+    S = img.shape[2:]
+    n = int(img[0, 0].size)
+    sp = [1.0, 1.0, 1.0]
+    threads = oracle.num_threads()
+    t_all = _median_time(lambda: oracle.geodesic3d_layout(img, mask, sp, 1e10, 1.0, 0.0, iters), runs)
+    slab_i = img[:, :, :slab_1t].copy()
+    slab_m = mask[:, :, :slab_1t].copy()
+    oracle.set_num_threads(1)
+    try:
+        t_one = _median_time(lambda: oracle.geodesic3d_layout(slab_i, slab_m, sp, 1e10, 1.0, 0.0, iters), 3)
+    finally:
+        oracle.set_num_threads(threads)
+    rate = n * 6 * iters / t_all
+    rate_1t = int(slab_i[0, 0].size) * 6 * iters / t_one
+    res = {"value": rate, "unit": "voxel-updates/s", "cores": threads, "kind": "port",
+           "sample": (f"the same {S[0]}x{S[1]}x{S[2]} call (lamb=1.0, iter={iters}): 1 warm-up, median of "
+                      f"{runs} = {t_all:.2f} s on {threads} threads; 1 thread: {slab_1t}x{S[1]}x{S[2]} slab, "
+                      f"median of 3 = {t_one:.2f} s"),
+           "value_1thread": rate_1t, "seconds_per_call": round(t_all, 3),
+           "what": "oracle/fg_oracle.c layout-faithful restatement of fastgeodis_cpu.cpp, -O3 -fopenmp "
+                   "-ffp-contract=off, bit-exact with the reference build on the golden fixtures"}
+    res.update(host_fields())
+    return res
 
 
 def c1_cpu_and_gpu():
-    """The small 2D call (512^2, C=1, lamb 1, iter 2) on the reference CPU path
-    (or the oracle port) and on the GPU: one warm-up, then the median of 10
-    CPU calls and of 50 GPU calls."""
-    import numpy as np
-
+    """The small 2D call (512^2, C=1, lamb 1, iter 2): the CPU restatement
+    (1 warm-up, median of 10, all threads and 1 thread) and the GPU (median of
+    50 calls, each synchronised)."""
     sys.path.insert(0, ROOT)
     import oracle
     from fastgeodis_amd import synthetic, workloads
 
     img, mask = synthetic.config1("cpu")
-    ref = oracle.reference_module()
-    if ref is not None:
-        kind, fn = "reference", lambda: ref.generalised_geodesic2d(img, mask, 1e10, 1.0, 0.0, 2)
-    else:
-        kind, fn = "port", lambda: oracle.py_geodesic2d(img.numpy(), mask.numpy(), 1e10, 1.0, 2)
-    fn()
-    times = []
-    for _ in range(10):
-        t0 = time.perf_counter()
-        fn()
-        times.append(time.perf_counter() - t0)
-    t_cpu = float(np.median(times))
+    img, mask = img.numpy(), mask.numpy()
+    threads = oracle.num_threads()
+    fn = lambda: oracle.geodesic2d_layout(img, mask, 1e10, 1.0, 0.0, 2)  # noqa: E731
+    t_cpu = _median_time(fn, 10)
+    oracle.set_num_threads(1)
+    try:
+        t_one = _median_time(fn, 10)
+    finally:
+        oracle.set_num_threads(threads)
     gpu = workloads.run_c1()
     n_upd = 512 * 512 * 4 * 2
     return {"workload": "1x1x512x512 blob image, 4 seeds, lamb 1, iter 2",
-            "cpu_ms": round(t_cpu * 1e3, 3), "cpu_vups": n_upd / t_cpu, "cpu_kind": kind,
-            "cpu_threads": int(os.environ.get("OMP_NUM_THREADS", str(os.cpu_count() or 1))),
+            "cpu_ms": round(t_cpu * 1e3, 3), "cpu_vups": n_upd / t_cpu, "cpu_kind": "port",
+            "cpu_threads": threads, "cpu_1thread_ms": round(t_one * 1e3, 3),
             "gpu_median_ms": round(gpu["median_ms"], 4), "gpu_vups": gpu["vups"]}
 
 
@@ -189,18 +212,28 @@ def main():
         torch.cuda.synchronize()
         log(f"[bench] warmup {w + 1}/{args.warmup} done")
 
-    FG.FastGeodisCpp.profile_enable(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for k in range(args.steps):
         out = step()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    FG.FastGeodisCpp.profile_enable(False)
-    launches, kern_ms, kern_bytes = FG.FastGeodisCpp.profile_collect()
+    # an expired halo wait in any timed call would have left the device's
+    # status word set: raise instead of reporting invalid work
+    FG.check_status()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = updates / (ms_per_step * 1e-3)
     log(f"[bench] {args.steps} steps: {ms_per_step:.3f} ms/step, {value / 1e9:.1f} Gvu/s")
+
+    # per-launch durations of the sweep kernel: HIP events around each launch
+    # on its own stream, in separate (untimed) calls
+    prof_steps = max(1, min(args.steps, 3))
+    FG.FastGeodisCpp.profile_enable(True)
+    for k in range(prof_steps):
+        step()
+    FG.FastGeodisCpp.profile_enable(False)
+    launches, kern_ms, kern_bytes = FG.FastGeodisCpp.profile_collect()
+    FG.check_status()
 
     avg_ms = kern_ms / max(launches, 1)
     bytes_per_launch = kern_bytes / max(launches, 1)

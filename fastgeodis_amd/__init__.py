@@ -52,6 +52,19 @@ def _load_native():
 FastGeodisCpp = _load_native()
 
 
+def _release_at_exit():
+    # frees pinned host memory and events before the HIP runtime tears down
+    try:
+        FastGeodisCpp.release()
+    except Exception:
+        pass
+
+
+import atexit  # noqa: E402
+
+atexit.register(_release_at_exit)
+
+
 def generalised_geodesic2d(
     image: torch.Tensor, softmask: torch.Tensor, v: float, lamb: float, iter: int = 2
 ):
@@ -192,6 +205,16 @@ def get_tuning():
     return {"engine": e, "halo": h, "max_groups": g, "shape": s}
 
 
+def check_status(device: Optional[int] = None) -> None:
+    """Synchronises the device's current stream and raises RuntimeError if a
+    launch on it reported an error (an expired bounded wait, whose call then
+    returned invalid distances) since the last check.  Every call also checks,
+    without synchronising, for errors left by earlier calls on its device."""
+    if device is None:
+        device = torch.cuda.current_device()
+    FastGeodisCpp.check_status(int(device))
+
+
 def native_library_paths() -> List[str]:
     """Paths of the native libraries this package loads."""
     import sysconfig
@@ -203,5 +226,5 @@ def native_library_paths() -> List[str]:
 __all__ = [
     "generalised_geodesic2d", "generalised_geodesic3d",
     "signed_generalised_geodesic2d", "signed_generalised_geodesic3d",
-    "GSF2d", "GSF3d", "Instance", "geodesic_batch", "set_tuning", "get_tuning",
+    "GSF2d", "GSF3d", "Instance", "geodesic_batch", "set_tuning", "get_tuning", "check_status",
 ]

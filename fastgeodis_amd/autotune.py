@@ -38,8 +38,32 @@ DEFAULT_CANDIDATES_2D: Tuple[Tuple[int, int, int], ...] = (
 _cache: Dict[str, dict] = {}
 
 
-def signature(shape: Sequence[int], channels: int) -> str:
-    return f"{len(shape)}d:{'x'.join(str(int(s)) for s in shape)}:c{int(channels)}"
+def variant_class(lamb: float) -> str:
+    """The kernel variant a lamb selects: "l1" (lamb == 1: the multiply-free
+    specialisation), "fast" (0 <= lamb <= 1: every step weight non-negative,
+    minimum over candidates) or "exact" (candidates folded one by one)."""
+    lamb = float(lamb)
+    if lamb == 1.0:
+        return "l1"
+    if 0.0 <= lamb <= 1.0:
+        return "fast"
+    return "exact"
+
+
+def device_tag(device="cuda") -> str:
+    """Device name and compute-unit count (tunings do not carry over between
+    devices of different size); "nodev" without a GPU."""
+    if not torch.cuda.is_available():
+        return "nodev"
+    p = torch.cuda.get_device_properties(torch.device(device))
+    name = getattr(p, "gcnArchName", "") or p.name
+    return f"{name.split(':')[0]}-{p.multi_processor_count}cu"
+
+
+def signature(shape: Sequence[int], channels: int, lamb: float = 1.0, iters: int = 1,
+              device="cuda") -> str:
+    return (f"{len(shape)}d:{'x'.join(str(int(s)) for s in shape)}:c{int(channels)}:"
+            f"{variant_class(lamb)}:it{int(iters)}:{device_tag(device)}")
 
 
 def _cache_file() -> Optional[str]:
@@ -80,7 +104,7 @@ def tune(shape: Sequence[int], channels: int = 1, spacing: Sequence[float] = (1.
     shape = [int(s) for s in shape]
     if len(shape) not in (2, 3):
         raise ValueError("shape must be (H, W) or (D, H, W)")
-    sig = signature(shape, channels)
+    sig = signature(shape, channels, lamb, iters, device)
     if use_cache and sig in load_cache():
         return _cache[sig]
     if candidates is None:

@@ -153,7 +153,7 @@ torch::Tensor run_device(const Call &c, const torch::Tensor &image_in, const tor
       rc = fg_gsf3d_f32(ip, mp, op, C, Cm, D, H, W, c.theta, sp, c.v, c.lg, c.iterations, wp, wsb, stream);
   }
   raise_for(rc);
-  if (sync_check_enabled()) raise_for(fg_status(wp, stream));
+  if (sync_check_enabled()) raise_for(fg_status(stream));
   return out;
 }
 
@@ -290,7 +290,7 @@ std::vector<torch::Tensor> geodesic_batch(const std::vector<torch::Tensor> &imag
   torch::Tensor ws = torch::empty({(int64_t)wsb}, ims[0].options().dtype(torch::kUInt8));
   void *stream = (void *)c10::hip::getCurrentHIPStream(dev.index()).stream();
   raise_for(fg_geodesic_batch_f32(work.data(), (int)work.size(), ws.data_ptr(), wsb, stream));
-  if (sync_check_enabled()) raise_for(fg_status(ws.data_ptr(), stream));
+  if (sync_check_enabled()) raise_for(fg_status(stream));
   return outs;
 }
 
@@ -314,6 +314,20 @@ std::tuple<int64_t, double, double> profile_collect() {
   return {n, ms, bytes};
 }
 
+// Synchronises the current stream of `device` and raises if a launch on that
+// device left an error in its status word since the last check.
+void check_status(int64_t device) {
+  c10::DeviceGuard guard(torch::Device(torch::kCUDA, (c10::DeviceIndex)device));
+  void *stream = (void *)c10::hip::getCurrentHIPStream((c10::DeviceIndex)device).stream();
+  raise_for(fg_status(stream));
+}
+
+int64_t pending_status(int64_t device) { return fg_pending_status((int)device); }
+
+void inject_status(int64_t device, int64_t code) { raise_for(fg_inject_status((int)device, (int)code)); }
+
+void set_wait_limit(int64_t spins) { raise_for(fg_set_wait_limit((int)spins)); }
+
 int64_t workspace_bytes(int64_t op, int64_t C, int64_t D, int64_t H, int64_t W) {
   return (int64_t)fg_workspace_bytes((int)op, (int)C, (int)D, (int)H, (int)W);
 }
@@ -334,4 +348,9 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("abi_version", []() { return fg_abi_version(); });
   m.def("profile_enable", &profile_enable, "Bracket tiled sweep launches with HIP events");
   m.def("profile_collect", &profile_collect, "(launches, total_ms, algorithmic_bytes) since enabling");
+  m.def("check_status", &check_status, "Synchronise the device's current stream and raise on a pending device error");
+  m.def("pending_status", &pending_status, "Read and clear a device's pending status code without synchronising");
+  m.def("inject_status", &inject_status, "Set a device's pending status from the host (error-path tests)");
+  m.def("set_wait_limit", &set_wait_limit, "Spin limit of the bounded halo waits for this host thread (0 = default)");
+  m.def("release", []() { fg_release(); }, "Free the library's pinned memory and events (process exit)");
 }

@@ -45,6 +45,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -217,7 +218,7 @@ struct Slab {  // one prefetched slice: image rows -1..R, distance rows 0..R-1
 // consecutive B.
 template <int NW, int R, int P, int CT, bool LAMB1, bool FAST, bool XST = false>
 __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
-    const SweepDesc *__restrict__ descs, int ndesc, int K, unsigned tag_base,
+    const SweepDesc *__restrict__ descs, int ndesc, SweepDesc one, int K, unsigned tag_base,
     unsigned long long *__restrict__ halo_base, int *__restrict__ status, int spin_limit) {
   // LDS slot = ring position of the step (a compile-time index in the
   // unrolled loops, so edge addresses are immediate offsets)
@@ -234,13 +235,20 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index: uniform
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
 
-  // descriptor owning this tile (binary search on tile0)
-  int lo = 0, hi = ndesc - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (descs[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
+  // descriptor owning this tile: the launch's only one (passed by value when
+  // descs is null) or a binary search on tile0 (uniform branches and loads,
+  // so the copy's fields stay in scalar registers)
+  SweepDesc Dd;
+  if (descs) {
+    int lo = 0, hi = ndesc - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (descs[mid].tile0 <= tile) lo = mid; else hi = mid - 1;
+    }
+    Dd = descs[lo];
+  } else {
+    Dd = one;
   }
-  const SweepDesc &Dd = descs[lo];
   gu64 *halo = (gu64 *)halo_base + Dd.halo0;
   const long long cs = Dd.cs, sA = Dd.sA, sB = Dd.sB;
   const int nA = Dd.nA, nB = Dd.nB, nE = Dd.nE, dir = Dd.dir;
@@ -273,7 +281,7 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   // every K steps; it stays within the sweep only when K is a positive
   // multiple of P.  A launch that breaks this is refused before any store.
   if (K <= 0 || K % P != 0) {
-    if (threadIdx.x == 0) __hip_atomic_store(status, FG_EINVAL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x == 0) __hip_atomic_store(status, FG_EINVAL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     return;
   }
   const int nsteps = nA - 1;
@@ -445,7 +453,7 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
       for (int r = 0; r < R; ++r) waiting = waiting || (need[r] && (unsigned)(x[r] >> 32) != tag);
       if (!waiting) break;
       if (++spins > spin_limit) {
-        __hip_atomic_store(status, FG_EWATCHDOG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(status, FG_EWATCHDOG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
         wd_tripped = true;
         break;
       }
@@ -606,15 +614,20 @@ __global__ void slice_kernel(SliceDesc D, int a, int aq) {
 
 constexpr int kXT = 64;
 
-__global__ __launch_bounds__(256) void xpose_kernel(const XposeDesc *__restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(256) void xpose_kernel(const XposeDesc *__restrict__ descs, int ndesc, XposeDesc one) {
   __shared__ float tile[kXT][kXT + 1];
   const int t = blockIdx.x;
-  int lo = 0, hi = ndesc - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (descs[mid].tile0 <= t) lo = mid; else hi = mid - 1;
+  XposeDesc X;
+  if (descs) {
+    int lo = 0, hi = ndesc - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (descs[mid].tile0 <= t) lo = mid; else hi = mid - 1;
+    }
+    X = descs[lo];
+  } else {
+    X = one;
   }
-  const XposeDesc &X = descs[lo];
   const int lt = t - X.tile0;
   const int plane = lt / X.tiles_dw, tt = lt - plane * X.tiles_dw;
   const int c = plane / X.H, h = plane - c * X.H;
@@ -705,14 +718,19 @@ __device__ __forceinline__ void ew_one(const EwDesc &E, long long i);
 // block-uniform search runs on scalar loads once per block), each thread 4
 // of them; the unsigned initialisation (every plain call) uses one 16-byte
 // load and store when the 4 elements are aligned.
-__global__ __launch_bounds__(256) void ew_kernel(const EwDesc *__restrict__ descs, int ndesc) {
+__global__ __launch_bounds__(256) void ew_kernel(const EwDesc *__restrict__ descs, int ndesc, EwDesc one) {
   const long long b = blockIdx.x;
-  int lo = 0, hi = ndesc - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (descs[mid].blk0 <= b) lo = mid; else hi = mid - 1;
+  EwDesc E;
+  if (descs) {
+    int lo = 0, hi = ndesc - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (descs[mid].blk0 <= b) lo = mid; else hi = mid - 1;
+    }
+    E = descs[lo];
+  } else {
+    E = one;
   }
-  const EwDesc &E = descs[lo];
   const long long i0 = (b - E.blk0) * kEwPerBlock + (long long)threadIdx.x * 4;
   if (i0 >= E.nelem) return;
   if (E.kind == EW_INIT_UNSIGNED && i0 + 4 <= E.nelem &&
@@ -780,8 +798,6 @@ __device__ __forceinline__ void ew_one(const EwDesc &E, long long i) {
   }
 }
 
-__global__ void fill_i32_kernel(int *p, int v) { *p = v; }
-
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -798,6 +814,9 @@ struct ProfRec {
   double bytes;
 };
 thread_local bool g_prof_on = false;
+// Checks of a pending halo value before a wait expires (0 = default); set by
+// fg_set_wait_limit.
+thread_local int g_wait_limit = 0;
 thread_local std::vector<ProfRec> g_prof;
 
 int fail(int code, const std::string &msg) {
@@ -1001,7 +1020,7 @@ struct Carver {
 // Recorded launch.  Descriptor arrays live in one host blob that is copied to
 // the workspace once per call, before any launch is issued.
 struct Cmd {
-  enum Type { TILED, SLICE, XPOSE, EW, STATUS } type;
+  enum Type { TILED, SLICE, XPOSE, EW } type;
   const void *kern = nullptr;
   int grid = 0, block = 0;
   size_t off = 0;  // descriptor offset in the blob
@@ -1011,6 +1030,11 @@ struct Cmd {
   SliceDesc slice{};  // SLICE: descriptor by value
   int a = 0, aq = 0;
   double alg_bytes = 0;  // TILED: algorithmic HBM bytes of the launch
+  // launches of one descriptor pass it by value (no blob entry)
+  bool by_value = false;
+  SweepDesc sweep1{};
+  XposeDesc xpose1{};
+  EwDesc ew1{};
 };
 
 struct Recorder {
@@ -1053,21 +1077,96 @@ int query_device(Device &dev) {
 // ordered per device: the host lock covers a whole replay, and a call on a
 // different stream than the previous one first waits for that call's end.
 // (Plain launches: the cooperative launch API adds ~15 us per launch.)
-struct DeviceOrder {
-  std::mutex m;
+//
+// The same per-device state holds the status word: one int in pinned,
+// host-mapped, coherent host memory that every launch of the device may set
+// (system-scope store) when a bounded wait expires or a launch is refused.
+// Nothing clears it on the device; the next call on the device (or
+// fg_status) reads and clears it from the host without synchronising, so an
+// expiry is reported at the latest by the call after the one it happened in.
+//
+// Descriptor blobs reach the device through pinned staging slots (a ring per
+// device): the copy is asynchronous, and a slot is reused only after the
+// event recorded behind its last copy has completed.
+struct StageSlot {
+  unsigned char *host = nullptr;
+  size_t cap = 0;
+  hipEvent_t copied{};
+  bool has_event = false;
+};
+constexpr int kStageSlots = 8;
+
+struct DeviceState {
+  std::mutex m;  // held for a whole replay
   bool has = false;
   hipStream_t last{};
   hipEvent_t done{};
+  int *status_host = nullptr;  // pinned word written by kernels
+  int *status_dev = nullptr;   // its device address
+  std::atomic<int> injected{0};  // set from the host (fg_inject_status)
+  StageSlot stage[kStageSlots];
+  int next_slot = 0;
 };
+
+std::mutex g_reg_m;
+std::map<int, std::unique_ptr<DeviceState>> g_reg;
+
+DeviceState &device_state(int id) {
+  std::lock_guard<std::mutex> g(g_reg_m);
+  auto &slot = g_reg[id];
+  if (!slot) slot.reset(new DeviceState());
+  return *slot;
+}
+
+// Allocates the device's status word on first use (the device is current).
+int ensure_status_word(DeviceState &s) {
+  std::lock_guard<std::mutex> g(s.m);
+  if (s.status_host) return FG_OK;
+  void *h = nullptr;
+  FG_HIP(hipHostMalloc(&h, 256, hipHostMallocMapped | hipHostMallocCoherent));
+  std::memset(h, 0, 256);
+  void *d = nullptr;
+  hipError_t e = hipHostGetDevicePointer(&d, h, 0);
+  if (e != hipSuccess) {
+    (void)hipHostFree(h);
+    return fail(FG_EHIP, std::string("hipHostGetDevicePointer: ") + hipGetErrorString(e));
+  }
+  s.status_host = (int *)h;
+  s.status_dev = (int *)d;
+  return FG_OK;
+}
+
+// Reads and clears the device's pending status (no synchronisation).
+int take_status(DeviceState &s) {
+  int v = s.injected.exchange(0);
+  if (s.status_host) {
+    const int w = __atomic_exchange_n(s.status_host, 0, __ATOMIC_SEQ_CST);
+    if (v == 0) v = w;
+  }
+  return v;
+}
+
+int status_error(int st, const char *when) {
+  if (st == FG_OK) return FG_OK;
+  if (st == FG_EWATCHDOG)
+    return fail(FG_EWATCHDOG, std::string(when) +
+                                  ": a tile's bounded halo wait expired (another persistent kernel may have held "
+                                  "compute units); the results of that call are invalid");
+  if (st == FG_EINVAL)
+    return fail(FG_EHIP, std::string(when) + ": a sweep launch was refused (halo depth not a multiple of its ring)");
+  return fail(FG_EHIP, std::string(when) + ": device status " + std::to_string(st));
+}
 
 // Holds a device's order for one replay; on every way out (errors included)
 // marks the end of what the replay enqueued on its stream.
 struct OrderedReplay {
-  DeviceOrder *ord = nullptr;
+  DeviceState *ord = nullptr;
   hipStream_t stream{};
   std::unique_lock<std::mutex> lock;
-  int begin(DeviceOrder &o, hipStream_t s) {
+  // order: the replay contains persistent sweeps and joins the device order
+  int begin(DeviceState &o, hipStream_t s, bool order) {
     lock = std::unique_lock<std::mutex>(o.m);
+    if (!order) return FG_OK;
     if (!o.has) {
       FG_HIP(hipEventCreateWithFlags(&o.done, hipEventDisableTiming));
       o.has = true;
@@ -1084,13 +1183,30 @@ struct OrderedReplay {
   }
 };
 
-DeviceOrder &device_order(int id) {
-  static std::mutex reg_m;
-  static std::map<int, std::unique_ptr<DeviceOrder>> reg;
-  std::lock_guard<std::mutex> g(reg_m);
-  auto &slot = reg[id];
-  if (!slot) slot.reset(new DeviceOrder());
-  return *slot;
+// Copies `bytes` of `src` to `dst` on `stream` through a pinned staging slot
+// of the device (the device's lock is held by the caller).
+int stage_copy(DeviceState &s, void *dst, const void *src, size_t bytes, hipStream_t stream) {
+  StageSlot &sl = s.stage[s.next_slot];
+  s.next_slot = (s.next_slot + 1) % kStageSlots;
+  if (sl.has_event) FG_HIP(hipEventSynchronize(sl.copied));
+  if (sl.cap < bytes) {
+    if (sl.host) FG_HIP(hipHostFree(sl.host));
+    sl.host = nullptr;
+    sl.cap = 0;
+    const size_t cap = std::max<size_t>(bytes, 64 << 10);
+    void *h = nullptr;
+    FG_HIP(hipHostMalloc(&h, cap, hipHostMallocDefault));
+    sl.host = (unsigned char *)h;
+    sl.cap = cap;
+  }
+  if (!sl.has_event) {
+    FG_HIP(hipEventCreateWithFlags(&sl.copied, hipEventDisableTiming));
+    sl.has_event = true;
+  }
+  std::memcpy(sl.host, src, bytes);
+  FG_HIP(hipMemcpyAsync(dst, sl.host, bytes, hipMemcpyHostToDevice, stream));
+  FG_HIP(hipEventRecord(sl.copied, stream));
+  return FG_OK;
 }
 
 // Occupancy-limited number of co-resident workgroups per kernel (cached per
@@ -1226,8 +1342,13 @@ struct Runner {
     if (!c.kern) return fail(FG_EINVAL, "no tiled kernel for this shape");
     c.grid = tiles;
     c.block = NW * 64;
-    c.off = rec.put(descs.data(), descs.size() * sizeof(SweepDesc));
     c.ndesc = (int)descs.size();
+    if (c.ndesc == 1) {
+      c.by_value = true;
+      c.sweep1 = descs[0];
+    } else {
+      c.off = rec.put(descs.data(), descs.size() * sizeof(SweepDesc));
+    }
     c.K = kk;
     c.tag_base = tag_base;
     c.alg_bytes = alg;
@@ -1255,8 +1376,13 @@ struct Runner {
     if (blocks == 0) return;
     Cmd c;
     c.type = Cmd::EW;
-    c.off = rec.put(ds.data(), ds.size() * sizeof(EwDesc));
     c.ndesc = (int)ds.size();
+    if (c.ndesc == 1) {
+      c.by_value = true;
+      c.ew1 = ds[0];
+    } else {
+      c.off = rec.put(ds.data(), ds.size() * sizeof(EwDesc));
+    }
     c.block = 256;
     c.grid = (int)blocks;  // < 2^31: 2^41 elements would not fit any device
     rec.cmds.push_back(c);
@@ -1273,8 +1399,13 @@ struct Runner {
     if (tiles == 0) return;
     Cmd c;
     c.type = Cmd::XPOSE;
-    c.off = rec.put(ds.data(), ds.size() * sizeof(XposeDesc));
     c.ndesc = (int)ds.size();
+    if (c.ndesc == 1) {
+      c.by_value = true;
+      c.xpose1 = ds[0];
+    } else {
+      c.off = rec.put(ds.data(), ds.size() * sizeof(XposeDesc));
+    }
     c.grid = tiles;
     c.block = 256;
     rec.cmds.push_back(c);
@@ -1458,36 +1589,32 @@ struct Runner {
     return FG_OK;
   }
 
-  int replay(hipStream_t stream, unsigned char *blob_dev, size_t blob_cap, int *status,
-             unsigned long long *halo) {
+  int replay(hipStream_t stream, unsigned char *blob_dev, size_t blob_cap, unsigned long long *halo) {
     if (rec.blob.size() > blob_cap) return fail(FG_EWORKSPACE, "descriptor area too small");
     // the halo base is only known now: patch fill descriptors
     for (auto &c : rec.cmds)
       if (c.type == Cmd::EW) {
-        EwDesc *ds = reinterpret_cast<EwDesc *>(rec.blob.data() + c.off);
+        EwDesc *ds = c.by_value ? &c.ew1 : reinterpret_cast<EwDesc *>(rec.blob.data() + c.off);
         for (int i = 0; i < c.ndesc; ++i)
           if (ds[i].kind == EW_FILL_U64) ds[i].u64 = halo;
       }
     bool tiled = false;
     for (auto &c : rec.cmds) tiled = tiled || c.type == Cmd::TILED;
+    DeviceState &ds = device_state(dev.id);
+    int *status = ds.status_dev;
     OrderedReplay order;
-    if (tiled) {
-      const int rc = order.begin(device_order(dev.id), stream);
-      if (rc) return rc;
-    }
-    if (!rec.blob.empty())
-      FG_HIP(hipMemcpyAsync(blob_dev, rec.blob.data(), rec.blob.size(), hipMemcpyHostToDevice, stream));
-    const int spin = 1 << 21;
+    int rc = order.begin(ds, stream, tiled);
+    if (rc) return rc;
+    if (!rec.blob.empty() && (rc = stage_copy(ds, blob_dev, rec.blob.data(), rec.blob.size(), stream))) return rc;
+    // the kernel expires a wait when a check finds a halo value pending after spin re-polls
+    const int spin = g_wait_limit > 0 ? g_wait_limit - 1 : (1 << 21);
     for (auto &c : rec.cmds) {
       switch (c.type) {
-        case Cmd::STATUS:
-          hipLaunchKernelGGL(fill_i32_kernel, dim3(1), dim3(1), 0, stream, status, 0);
-          break;
         case Cmd::TILED: {
-          const SweepDesc *dp = reinterpret_cast<const SweepDesc *>(blob_dev + c.off);
+          const SweepDesc *dp = c.by_value ? nullptr : reinterpret_cast<const SweepDesc *>(blob_dev + c.off);
           int ndesc = c.ndesc, k = c.K, sp = spin;
           unsigned tb = c.tag_base;
-          void *args[] = {(void *)&dp, &ndesc, &k, &tb, (void *)&halo, (void *)&status, &sp};
+          void *args[] = {(void *)&dp, &ndesc, (void *)&c.sweep1, &k, &tb, (void *)&halo, (void *)&status, &sp};
           ProfRec pr{};
           if (g_prof_on) {
             FG_HIP(hipEventCreate(&pr.start));
@@ -1506,13 +1633,13 @@ struct Runner {
           hipLaunchKernelGGL(slice_kernel, dim3(c.grid), dim3(c.block), 0, stream, c.slice, c.a, c.aq);
           break;
         case Cmd::XPOSE: {
-          const XposeDesc *dp = reinterpret_cast<const XposeDesc *>(blob_dev + c.off);
-          hipLaunchKernelGGL(xpose_kernel, dim3(c.grid), dim3(c.block), 0, stream, dp, c.ndesc);
+          const XposeDesc *dp = c.by_value ? nullptr : reinterpret_cast<const XposeDesc *>(blob_dev + c.off);
+          hipLaunchKernelGGL(xpose_kernel, dim3(c.grid), dim3(c.block), 0, stream, dp, c.ndesc, c.xpose1);
           break;
         }
         case Cmd::EW: {
-          const EwDesc *dp = reinterpret_cast<const EwDesc *>(blob_dev + c.off);
-          hipLaunchKernelGGL(ew_kernel, dim3(c.grid), dim3(c.block), 0, stream, dp, c.ndesc);
+          const EwDesc *dp = c.by_value ? nullptr : reinterpret_cast<const EwDesc *>(blob_dev + c.off);
+          hipLaunchKernelGGL(ew_kernel, dim3(c.grid), dim3(c.block), 0, stream, dp, c.ndesc, c.ew1);
           break;
         }
       }
@@ -1612,7 +1739,6 @@ size_t blob_bytes_for(const std::vector<OpSpec> &ops) {
 }
 
 struct Layout {
-  int *status;
   unsigned char *blob;
   size_t blob_cap;
   unsigned long long *halo;
@@ -1620,7 +1746,6 @@ struct Layout {
 };
 
 void layout_common(const std::vector<OpSpec> &ops, int K, Layout &L, Carver &cv) {
-  L.status = (int *)cv.take(sizeof(int));
   L.blob_cap = blob_bytes_for(ops);
   L.blob = (unsigned char *)cv.take(L.blob_cap);
   L.halo_cap = halo_entries_for(ops, K);
@@ -1670,6 +1795,10 @@ int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream) 
   Runner run{};
   int rc = query_device(run.dev);
   if (rc) return rc;
+  // an expiry left by an earlier call on this device is reported here
+  DeviceState &dstate = device_state(run.dev.id);
+  if ((rc = ensure_status_word(dstate))) return rc;
+  if ((rc = status_error(take_status(dstate), "an earlier call on this device"))) return rc;
   run.K = K;
   run.engine = g_tune.engine;
   run.max_groups = g_tune.max_groups;
@@ -1691,12 +1820,7 @@ int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream) 
   }
   if (!cv.fits()) return fail(FG_EWORKSPACE, "workspace too small");
 
-  // the status word and the halo tags start from zero on every call
-  {
-    Cmd c;
-    c.type = Cmd::STATUS;
-    run.rec.cmds.push_back(c);
-  }
+  // the halo tags start from zero on every call
   run.record_zero_halo();
 
   // image transposes (once per op)
@@ -1795,7 +1919,7 @@ int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream) 
     if ((rc = run.record_run(xs2))) return rc;
   }
   run.record_ew(fin);
-  return run.replay(stream, L.blob, L.blob_cap, L.status, L.halo);
+  return run.replay(stream, L.blob, L.blob_cap, L.halo);
 }
 
 OpSpec spec2d(const float *image, const float *mask, float *out, int C, int Cm, int H, int W, int op,
@@ -1830,7 +1954,7 @@ int run_single(OpSpec o, void *ws, size_t wsb, void *stream) {
 
 extern "C" {
 
-int fg_abi_version(void) { return 1; }
+int fg_abi_version(void) { return 2; }
 
 const char *fg_last_error(void) { return g_err.c_str(); }
 
@@ -1965,16 +2089,55 @@ int fg_profile_collect(int *launches, double *total_ms, double *alg_bytes) {
   return rc;
 }
 
-int fg_status(const void *workspace, void *stream) {
-  if (!workspace) return fail(FG_EINVAL, "null workspace");
-  hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+int fg_status(void *stream) {
+  int id = 0;
+  hipError_t e = hipGetDevice(&id);
+  if (e != hipSuccess) return fail(FG_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  e = hipStreamSynchronize((hipStream_t)stream);
   if (e != hipSuccess) return fail(FG_EHIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
-  int st = 0;
-  e = hipMemcpy(&st, workspace, sizeof(int), hipMemcpyDeviceToHost);
-  if (e != hipSuccess) return fail(FG_EHIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
-  if (st == FG_EWATCHDOG) return fail(FG_EWATCHDOG, "a tile's halo wait expired (watchdog)");
-  if (st == FG_EINVAL) return fail(FG_EINVAL, "a sweep launch was refused: halo depth not a multiple of its slab ring");
+  return status_error(take_status(device_state(id)), "status");
+}
+
+int fg_pending_status(int device) {
+  if (device < 0) return fail(FG_EINVAL, "device must be >= 0");
+  return take_status(device_state(device));
+}
+
+int fg_inject_status(int device, int code) {
+  if (device < 0) return fail(FG_EINVAL, "device must be >= 0");
+  device_state(device).injected.store(code);
   return FG_OK;
+}
+
+int fg_set_wait_limit(int spins) {
+  if (spins < 0) return fail(FG_EINVAL, "wait limit must be >= 0");
+  g_wait_limit = spins;
+  return FG_OK;
+}
+
+void fg_release(void) {
+  int rc = fg_profile_collect(nullptr, nullptr, nullptr);
+  (void)rc;
+  std::lock_guard<std::mutex> g(g_reg_m);
+  for (auto &kv : g_reg) {
+    DeviceState &s = *kv.second;
+    std::lock_guard<std::mutex> l(s.m);
+    if (s.has) (void)hipEventSynchronize(s.done);
+    for (auto &sl : s.stage) {
+      if (sl.has_event) {
+        (void)hipEventSynchronize(sl.copied);
+        (void)hipEventDestroy(sl.copied);
+        sl.has_event = false;
+      }
+      if (sl.host) (void)hipHostFree(sl.host);
+      sl.host = nullptr;
+      sl.cap = 0;
+    }
+    if (s.has) (void)hipEventDestroy(s.done);
+    s.has = false;
+    if (s.status_host) (void)hipHostFree(s.status_host);
+    s.status_host = s.status_dev = nullptr;
+  }
 }
 
 }  // extern "C"

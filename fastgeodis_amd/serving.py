@@ -56,10 +56,17 @@ class Click:
     voxels: Sequence[Tuple[int, int, int]]
     foreground: bool
     arrival: float = field(default_factory=time.perf_counter)
+    # the submitting thread's current stream (set by Server.submit): the
+    # result is handed over ordered before later work on it
+    stream: Optional[torch.cuda.Stream] = None
 
 
 @dataclass
 class Result:
+    """distance is complete when the future resolves.  It is recorded as in
+    use on the stream that was current in the submitting thread (so freeing it
+    there is safe), and that stream waits for the batch that produced it; on
+    any other stream, call ``distance.record_stream(stream)`` before use."""
     distance: torch.Tensor         # (1, 1, D, H, W) signed distance, on device
     latency_s: float
     batch_size: int
@@ -100,6 +107,13 @@ class Server:
             raise ValueError("session image must be (1, C, D, H, W)")
         image = image.to(self.device, torch.float32).contiguous()
         mask = torch.full((1, 1) + tuple(image.shape[2:]), 0.5, device=self.device)
+        # both were written on the caller's stream; the server's stream reads
+        # and edits them from now on
+        ready = torch.cuda.Event()
+        ready.record(torch.cuda.current_stream(self.device))
+        self._stream.wait_event(ready)
+        image.record_stream(self._stream)
+        mask.record_stream(self._stream)
         self.sessions[sid] = Session(image, mask, tuple(float(s) for s in spacing))
 
     def request_bytes(self, sid: int) -> int:
@@ -117,6 +131,8 @@ class Server:
             raise KeyError(f"unknown session {click.session}")
         if self.request_bytes(click.session) > self.budget:
             raise ValueError("request larger than the HBM budget")
+        if click.stream is None:
+            click.stream = torch.cuda.current_stream(self.device)
         fut: Future = Future()
         with self._cv:
             if self._error is not None:
@@ -222,6 +238,10 @@ class Server:
                 pending.popleft()
                 t = time.perf_counter()
                 for (click, fut), out in zip(b, outs):
+                    consumer = click.stream
+                    if consumer is not None and consumer != self._stream:
+                        consumer.wait_event(ev)
+                        out.record_stream(consumer)
                     fut.set_result(Result(out, t - click.arrival, len(b)))
 
 

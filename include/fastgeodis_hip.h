@@ -60,10 +60,32 @@ const char *fg_last_error(void);
  * (fg_tuning_set) of the calling host thread. */
 size_t fg_workspace_bytes(int op, int C, int D, int H, int W);
 
-/* Synchronises `stream` and reads the status word at the start of
- * `workspace` that the last call on it left: FG_OK, or FG_EWATCHDOG when a
- * bounded device-side wait expired (the results of that call are invalid). */
-int fg_status(const void *workspace, void *stream);
+/* ---- device status (no counterpart in the reference, whose kernels report
+ * nothing, fastgeodis_cuda.cu:232-260, 541-570).  Each device has one
+ * persistent status word in pinned host memory.  A launch sets it when a
+ * bounded device-side wait expires (FG_EWATCHDOG: a tile never arrived, the
+ * results of that call are invalid) or a launch is refused.  Every compute
+ * call first reads and clears the word of its device, without
+ * synchronising, and fails with that code when it was set, so an expiry is
+ * reported by the next call at the latest.
+ *
+ * fg_status synchronises `stream` and then reads and clears the word of the
+ * current device: FG_OK, FG_EWATCHDOG or FG_EHIP.  fg_pending_status reads
+ * and clears the word of `device` without any HIP call.  fg_inject_status
+ * sets a device's pending status from the host (tests of the error path).
+ * fg_set_wait_limit sets how many checks of a pending halo value a bounded
+ * wait makes before it expires, for later calls on this host thread (0 = the
+ * default, about 2^21; 1 expires at the first check that finds a value
+ * missing). */
+int fg_status(void *stream);
+int fg_pending_status(int device);
+int fg_inject_status(int device, int code);
+int fg_set_wait_limit(int spins);
+
+/* Waits for outstanding work of the library's events and frees its pinned
+ * host memory and events (call once at process exit; later calls
+ * re-create what they need). */
+void fg_release(void);
 
 /* Replaces generalised_geodesic2d_cuda (reference fastgeodis.h:38-44,
  * fastgeodis_cuda.cu:278-318): out = transform of fl(v*mask) after

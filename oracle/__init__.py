@@ -62,7 +62,13 @@ def _load():
     for name in ("fgo_geodesic2d", "fgo_geodesic3d", "fgo_signed_geodesic2d",
                  "fgo_signed_geodesic3d", "fgo_gsf2d", "fgo_gsf3d"):
         getattr(lib, name).restype = ctypes.c_int
+    lib.fgo_geodesic2d_layout.argtypes = [p, p, p, i, i, i, i, f, f, f, i]
+    lib.fgo_geodesic3d_layout.argtypes = [p, p, p, i, i, i, i, i, p, f, f, f, i]
+    lib.fgo_geodesic2d_layout.restype = ctypes.c_int
+    lib.fgo_geodesic3d_layout.restype = ctypes.c_int
     lib.fgo_num_threads.restype = ctypes.c_int
+    lib.fgo_set_num_threads.argtypes = [i]
+    lib.fgo_set_num_threads.restype = None
     _lib = lib
     return lib
 
@@ -93,6 +99,33 @@ def _check(rc):
 
 def num_threads() -> int:
     return _load().fgo_num_threads()
+
+
+def set_num_threads(n: int) -> None:
+    """OpenMP thread count of later oracle calls."""
+    _load().fgo_set_num_threads(int(n))
+
+
+# ---- layout-faithful forms: the CPU baseline ------------------------------
+# Same results as geodesic2d/3d; they copy the image and distance into each
+# pass's layout like the reference's transposes (fastgeodis_cpu.cpp:180-195,
+# 361-401), so their speed reflects the reference's CPU path.
+
+def geodesic2d_layout(image, mask, v, l_grad, l_eucl, iterations):
+    image, mask, out = _prep(image, mask, 2)
+    _, C, H, W = image.shape
+    _check(_load().fgo_geodesic2d_layout(_ptr(image), _ptr(mask), _ptr(out), C, mask.shape[1],
+                                         H, W, v, l_grad, l_eucl, int(iterations)))
+    return out
+
+
+def geodesic3d_layout(image, mask, spacing, v, l_grad, l_eucl, iterations):
+    image, mask, out = _prep(image, mask, 3)
+    _, C, D, H, W = image.shape
+    s = _spacing(spacing)
+    _check(_load().fgo_geodesic3d_layout(_ptr(image), _ptr(mask), _ptr(out), C, mask.shape[1],
+                                         D, H, W, _ptr(s), v, l_grad, l_eucl, int(iterations)))
+    return out
 
 
 # ---- FastGeodisCpp-level entry points (explicit l_grad / l_eucl) ---------

@@ -272,3 +272,122 @@ int fgo_num_threads(void) {
   return 1;
 #endif
 }
+
+/* Sets the OpenMP thread count of later calls (n >= 1). */
+void fgo_set_num_threads(int n) {
+#ifdef _OPENMP
+  if (n >= 1) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+/* ---- layout-faithful forms (the CPU baseline) ----------------------------
+ * The reference runs every pass on a contiguous copy whose outermost spatial
+ * axis is the sweep axis: per 3D iteration it materialises the image and the
+ * distance in (H, D, W) for the H pass, in (W, H, D) for the W pass and back
+ * in (D, H, W) (transpose + contiguous at fastgeodis_cpu.cpp:361-401); per 2D
+ * iteration in (W, H) for the column pass and back (:180-195).  These forms
+ * do the same copies, OpenMP-parallel, and run the same pass_pair on
+ * contiguous rows, so their timing reflects the reference's memory traffic.
+ * Results equal fgo_geodesic2d/3d bit for bit (copies only move data). */
+
+/* dst (C, n0, n1, n2) contiguous; dst[c][i][j][k] = src[c*cs + i*s0 + j*s1 + k*s2] */
+static void permute_copy(const float *src, float *dst, int C, ptrdiff_t cs,
+                         int n0, int n1, int n2, ptrdiff_t s0, ptrdiff_t s1,
+                         ptrdiff_t s2) {
+  const int T = 32;
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int c = 0; c < C; ++c) {
+    for (int i = 0; i < n0; ++i) {
+      const float *sp = src + (ptrdiff_t)c * cs + (ptrdiff_t)i * s0;
+      float *dp = dst + ((ptrdiff_t)c * n0 + i) * n1 * (ptrdiff_t)n2;
+      for (int j0 = 0; j0 < n1; j0 += T)
+        for (int k0 = 0; k0 < n2; k0 += T)
+          for (int j = j0; j < j0 + T && j < n1; ++j)
+            for (int k = k0; k < k0 + T && k < n2; ++k)
+              dp[(ptrdiff_t)j * n2 + k] = sp[(ptrdiff_t)j * s1 + (ptrdiff_t)k * s2];
+    }
+  }
+}
+
+int fgo_geodesic3d_layout(const float *img, const float *mask, float *out,
+                          int C, int Cm, int D, int H, int W,
+                          const float spacing[3], float v, float l_grad,
+                          float l_eucl, int iters) {
+  if (C < 1 || Cm < 1 || D < 1 || H < 1 || W < 1) return -1;
+  const size_t n = (size_t)D * H * W;
+  float sq[3];
+  for (int k = 0; k < 3; ++k) sq[k] = spacing[k] * spacing[k];
+  float ldD[9], ldH[9], ldW[9];
+  table3d(sq[0], sq[1], sq[2], ldD);
+  table3d(sq[1], sq[0], sq[2], ldH);
+  table3d(sq[2], sq[1], sq[0], ldW);
+  init_dist(mask, out, (size_t)Cm * n, v);
+  float *ia = (float *)malloc((size_t)C * n * sizeof(float));
+  float *ib = (float *)malloc((size_t)C * n * sizeof(float));
+  float *da = (float *)malloc(n * sizeof(float));
+  float *db = (float *)malloc(n * sizeof(float));
+  if (!ia || !ib || !da || !db) {
+    free(ia); free(ib); free(da); free(db);
+    return -2;
+  }
+  const ptrdiff_t cs = (ptrdiff_t)n;
+  /* image_local = image.clone(); distance channel 0 in da (D, H, W) */
+  memcpy(ia, img, (size_t)C * n * sizeof(float));
+  memcpy(da, out, n * sizeof(float));
+  for (int it = 0; it < iters; ++it) {
+    /* D pass on (D, H, W) */
+    pass_pair(ia, da, C, cs, D, (ptrdiff_t)H * W, H, W, W, 1, ldD, l_grad, l_eucl);
+    /* (D, H, W) -> (H, D, W) */
+    permute_copy(ia, ib, C, cs, H, D, W, W, (ptrdiff_t)H * W, 1);
+    permute_copy(da, db, 1, cs, H, D, W, W, (ptrdiff_t)H * W, 1);
+    pass_pair(ib, db, C, cs, H, (ptrdiff_t)D * W, D, W, W, 1, ldH, l_grad, l_eucl);
+    /* (H, D, W) -> (W, H, D) */
+    permute_copy(ib, ia, C, cs, W, H, D, 1, (ptrdiff_t)D * W, W);
+    permute_copy(db, da, 1, cs, W, H, D, 1, (ptrdiff_t)D * W, W);
+    pass_pair(ia, da, C, cs, W, (ptrdiff_t)H * D, H, D, D, 1, ldW, l_grad, l_eucl);
+    /* (W, H, D) -> (D, H, W) */
+    permute_copy(ia, ib, C, cs, D, H, W, 1, D, (ptrdiff_t)H * D);
+    permute_copy(da, db, 1, cs, D, H, W, 1, D, (ptrdiff_t)H * D);
+    float *t = ia; ia = ib; ib = t;
+    t = da; da = db; db = t;
+  }
+  memcpy(out, da, n * sizeof(float));
+  free(ia); free(ib); free(da); free(db);
+  return 0;
+}
+
+int fgo_geodesic2d_layout(const float *img, const float *mask, float *out,
+                          int C, int Cm, int H, int W, float v, float l_grad,
+                          float l_eucl, int iters) {
+  if (C < 1 || Cm < 1 || H < 1 || W < 1) return -1;
+  const size_t n = (size_t)H * W;
+  const float r2 = sqrtf(2.0f);
+  const float ld[9] = {0.f, 0.f, 0.f, r2, 1.0f, r2, 0.f, 0.f, 0.f};
+  init_dist(mask, out, (size_t)Cm * n, v);
+  float *ia = (float *)malloc((size_t)C * n * sizeof(float));
+  float *ib = (float *)malloc((size_t)C * n * sizeof(float));
+  float *da = (float *)malloc(n * sizeof(float));
+  float *db = (float *)malloc(n * sizeof(float));
+  if (!ia || !ib || !da || !db) {
+    free(ia); free(ib); free(da); free(db);
+    return -2;
+  }
+  const ptrdiff_t cs = (ptrdiff_t)n;
+  memcpy(ia, img, (size_t)C * n * sizeof(float));
+  memcpy(da, out, n * sizeof(float));
+  for (int it = 0; it < iters; ++it) {
+    pass_pair(ia, da, C, cs, H, W, 1, 0, W, 1, ld, l_grad, l_eucl);
+    /* (H, W) -> (W, H) */
+    permute_copy(ia, ib, C, cs, 1, W, H, 0, 1, W);
+    permute_copy(da, db, 1, cs, 1, W, H, 0, 1, W);
+    pass_pair(ib, db, C, cs, W, H, 1, 0, H, 1, ld, l_grad, l_eucl);
+    /* (W, H) -> (H, W) */
+    permute_copy(ib, ia, C, cs, 1, H, W, 0, 1, H);
+    permute_copy(db, da, 1, cs, 1, H, W, 0, 1, H);
+  }
+  memcpy(out, da, n * sizeof(float));
+  free(ia); free(ib); free(da); free(db);
+  return 0;
+}

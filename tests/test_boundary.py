@@ -42,7 +42,7 @@ def test_library_exports_every_declared_symbol(lib):
 
 def test_abi_version_and_errors(lib):
     lib.fg_abi_version.restype = ctypes.c_int
-    assert lib.fg_abi_version() == 1
+    assert lib.fg_abi_version() == 2
     lib.fg_last_error.restype = ctypes.c_char_p
     f = lib.fg_geodesic2d_f32
     f.restype = ctypes.c_int
@@ -51,6 +51,31 @@ def test_abi_version_and_errors(lib):
     rc = f(None, None, None, 1, 1, 8, 8, 1e10, 1.0, 0.0, 2, None, 0, None)
     assert rc == -1
     assert b"null" in lib.fg_last_error()
+
+
+def test_pending_status_is_read_and_cleared(lib):
+    """The per-device status word: a value set from the host is returned once by
+    fg_pending_status and then cleared (host logic only, no HIP call)."""
+    lib.fg_pending_status.restype = ctypes.c_int
+    lib.fg_pending_status.argtypes = [ctypes.c_int]
+    lib.fg_inject_status.restype = ctypes.c_int
+    lib.fg_inject_status.argtypes = [ctypes.c_int, ctypes.c_int]
+    assert lib.fg_pending_status(3) == 0
+    assert lib.fg_inject_status(3, -3) == 0
+    assert lib.fg_pending_status(2) == 0  # other devices unaffected
+    assert lib.fg_pending_status(3) == -3
+    assert lib.fg_pending_status(3) == 0
+    assert lib.fg_inject_status(-1, -3) == -1
+    lib.fg_set_wait_limit.restype = ctypes.c_int
+    lib.fg_set_wait_limit.argtypes = [ctypes.c_int]
+    assert lib.fg_set_wait_limit(-5) == -1
+    assert lib.fg_set_wait_limit(0) == 0
+
+
+def test_pending_status_through_the_glue():
+    FG.FastGeodisCpp.inject_status(5, -3)
+    assert FG.FastGeodisCpp.pending_status(5) == -3
+    assert FG.FastGeodisCpp.pending_status(5) == 0
 
 
 def test_workspace_query(lib):

@@ -263,7 +263,7 @@ def test_c_abi_direct(device):
     q.argtypes = [ctypes.c_int] * 5
     st = lib.fg_status
     st.restype = ctypes.c_int
-    st.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+    st.argtypes = [ctypes.c_void_p]
     rng = np.random.default_rng(3)
     shape = (21, 66, 70)
     img = rng.random((1, 2, *shape), dtype=np.float32)
@@ -279,6 +279,6 @@ def test_c_abi_direct(device):
     rc = f(ti.data_ptr(), tm.data_ptr(), out.data_ptr(), 2, 1, *shape, sp, 1e10, float(lg), float(le), 3,
            ws.data_ptr(), wsb, stream)
     assert rc == 0
-    assert st(ws.data_ptr(), stream) == 0
+    assert st(stream) == 0
     want = oracle.geodesic3d(img, mask, [2.0, 1.0, 0.5], 1e10, lg, le, 3)
     assert bits_equal(out.cpu().numpy(), want)

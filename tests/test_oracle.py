@@ -204,3 +204,25 @@ def test_reference_euclidean_band():
         exp = math.sqrt(nd * b * b)
         tol = 10 if nd == 2 else 100
         assert exp - tol < out.max() < exp + tol
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_layout_faithful_forms_equal_the_oracle(seed):
+    """The CPU-baseline forms (per-pass contiguous copies like the reference's
+    transposes) return the oracle's bits."""
+    rng = np.random.default_rng(300 + seed)
+    C = int(rng.integers(1, 4))
+    lamb = float(rng.choice([0.0, 0.5, 1.0]))
+    lg, le = oracle.wrapper_weights(lamb)
+    it = int(rng.integers(1, 4))
+    shape2 = [int(x) for x in rng.integers(1, 70, size=2)]
+    img = rng.random((1, C, *shape2), dtype=np.float32)
+    mask = (rng.random((1, 1, *shape2)) > 0.05).astype(np.float32)
+    assert bits_equal(oracle.geodesic2d_layout(img, mask, 1e10, lg, le, it),
+                      oracle.geodesic2d(img, mask, 1e10, lg, le, it))
+    shape3 = [int(x) for x in rng.integers(1, 30, size=3)]
+    sp = [float(x) for x in rng.uniform(0.4, 3.0, size=3)]
+    img = rng.random((1, C, *shape3), dtype=np.float32)
+    mask = (rng.random((1, 1, *shape3)) > 0.05).astype(np.float32)
+    assert bits_equal(oracle.geodesic3d_layout(img, mask, sp, 1e10, lg, le, it),
+                      oracle.geodesic3d(img, mask, sp, 1e10, lg, le, it))
