@@ -1096,6 +1096,14 @@ struct StageSlot {
 };
 constexpr int kStageSlots = 8;
 
+// Scratch of the no-allocation entry point (fg_geodesic3d_f32_noalloc): one
+// static device array per device, held from the library's load on that device
+// until the process ends.  Bytes [0, 256) hold that entry point's status word
+// while the device has no pinned status word; the rest is its workspace.
+// Host code uses the address from hipGetSymbolAddress, never the name.
+constexpr size_t kNoAllocScratchBytes = size_t(256) << 20;
+constexpr size_t kNoAllocStatusBytes = 256;
+
 struct DeviceState {
   std::mutex m;  // held for a whole replay
   bool has = false;
@@ -1106,6 +1114,7 @@ struct DeviceState {
   std::atomic<int> injected{0};  // set from the host (fg_inject_status)
   StageSlot stage[kStageSlots];
   int next_slot = 0;
+  unsigned char *scratch = nullptr;  // device address of the no-allocation scratch (per device)
 };
 
 std::mutex g_reg_m;
@@ -1163,10 +1172,12 @@ struct OrderedReplay {
   DeviceState *ord = nullptr;
   hipStream_t stream{};
   std::unique_lock<std::mutex> lock;
-  // order: the replay contains persistent sweeps and joins the device order
-  int begin(DeviceState &o, hipStream_t s, bool order) {
+  // order: the replay contains persistent sweeps and joins the device order;
+  // create: the order's event may be created here (not for callers that
+  // allocate nothing, which then join only an existing order)
+  int begin(DeviceState &o, hipStream_t s, bool order, bool create = true) {
     lock = std::unique_lock<std::mutex>(o.m);
-    if (!order) return FG_OK;
+    if (!order || (!o.has && !create)) return FG_OK;
     if (!o.has) {
       FG_HIP(hipEventCreateWithFlags(&o.done, hipEventDisableTiming));
       o.has = true;
@@ -1589,7 +1600,10 @@ struct Runner {
     return FG_OK;
   }
 
-  int replay(hipStream_t stream, unsigned char *blob_dev, size_t blob_cap, unsigned long long *halo) {
+  // status: the status word the launches write; no_alloc: create nothing on the
+  // host side (no pinned staging, no events) and pass every descriptor by value
+  int replay(hipStream_t stream, unsigned char *blob_dev, size_t blob_cap, unsigned long long *halo, int *status,
+             bool no_alloc) {
     if (rec.blob.size() > blob_cap) return fail(FG_EWORKSPACE, "descriptor area too small");
     // the halo base is only known now: patch fill descriptors
     for (auto &c : rec.cmds)
@@ -1601,9 +1615,10 @@ struct Runner {
     bool tiled = false;
     for (auto &c : rec.cmds) tiled = tiled || c.type == Cmd::TILED;
     DeviceState &ds = device_state(dev.id);
-    int *status = ds.status_dev;
+    if (no_alloc && !rec.blob.empty())
+      return fail(FG_EINVAL, "internal: a launch of this call needs a descriptor array");
     OrderedReplay order;
-    int rc = order.begin(ds, stream, tiled);
+    int rc = order.begin(ds, stream, tiled, !no_alloc);
     if (rc) return rc;
     if (!rec.blob.empty() && (rc = stage_copy(ds, blob_dev, rec.blob.data(), rec.blob.size(), stream))) return rc;
     // the kernel expires a wait when a check finds a halo value pending after spin re-polls
@@ -1779,7 +1794,12 @@ int validate(const OpSpec &o) {
   return FG_OK;
 }
 
-int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream) {
+struct RunOpts {
+  bool no_alloc = false;           // create nothing (no pinned status word, staging or events)
+  int *fallback_status = nullptr;  // status word to use while the device has no pinned one
+};
+
+int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream, const RunOpts &opt = RunOpts()) {
   for (auto &o : ops) {
     int rc = validate(o);
     if (rc) return rc;
@@ -1797,8 +1817,10 @@ int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream) 
   if (rc) return rc;
   // an expiry left by an earlier call on this device is reported here
   DeviceState &dstate = device_state(run.dev.id);
-  if ((rc = ensure_status_word(dstate))) return rc;
+  if (!opt.no_alloc && (rc = ensure_status_word(dstate))) return rc;
   if ((rc = status_error(take_status(dstate), "an earlier call on this device"))) return rc;
+  int *status = dstate.status_dev ? dstate.status_dev : opt.fallback_status;
+  if (!status) return fail(FG_EINVAL, "internal: no status word");
   run.K = K;
   run.engine = g_tune.engine;
   run.max_groups = g_tune.max_groups;
@@ -1919,7 +1941,7 @@ int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream) 
     if ((rc = run.record_run(xs2))) return rc;
   }
   run.record_ew(fin);
-  return run.replay(stream, L.blob, L.blob_cap, L.halo);
+  return run.replay(stream, L.blob, L.blob_cap, L.halo, status, opt.no_alloc);
 }
 
 OpSpec spec2d(const float *image, const float *mask, float *out, int C, int Cm, int H, int W, int op,
@@ -1945,6 +1967,27 @@ int run_single(OpSpec o, void *ws, size_t wsb, void *stream) {
   return run_ops(ops, ws, wsb, (hipStream_t)stream);
 }
 
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+// Device address of this device's scratch array (the device is current),
+// looked up once per device.
+int noalloc_scratch(DeviceState &ds, unsigned char **out) {
+  std::lock_guard<std::mutex> g(ds.m);
+  if (!ds.scratch) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(FG_EHIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    ds.scratch = (unsigned char *)p;
+  }
+  *out = ds.scratch;
+  return FG_OK;
+}
 
 }  // namespace
 
@@ -2015,6 +2058,35 @@ int fg_gsf3d_f32(const float *image, const float *mask, float *out, int C, int C
       spec3d(image, mask, out, C, Cm, D, H, W, spacing, FG_OP_GSF, iterations, v, lambda, 0.f, theta),
       workspace, workspace_bytes, stream);
 }
+
+int fg_geodesic3d_f32_noalloc(const float *image, const float *mask, float *out, int C, int Cm, int D, int H,
+                              int W, const float spacing[3], float v, float l_grad, float l_eucl, int iterations,
+                              void *stream) {
+  if (!spacing) return fail(FG_EINVAL, "spacing must have 3 entries");
+  std::vector<OpSpec> ops{
+      spec3d(image, mask, out, C, Cm, D, H, W, spacing, FG_OP_UNSIGNED, iterations, v, l_grad, l_eucl, 0.f)};
+  int rc = validate(ops[0]);
+  if (rc) return rc;
+  const size_t need = workspace_for(ops, current_halo());
+  if (need > kNoAllocScratchBytes - kNoAllocStatusBytes)
+    return fail(FG_EWORKSPACE, "fg_geodesic3d_f32_noalloc: this volume needs " + std::to_string(need) +
+                                   " bytes of scratch; the no-allocation entry point holds " +
+                                   std::to_string(kNoAllocScratchBytes - kNoAllocStatusBytes) +
+                                   " (use fg_geodesic3d_f32 with a workspace)");
+  int id = 0;
+  hipError_t e = hipGetDevice(&id);
+  if (e != hipSuccess) return fail(FG_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  DeviceState &ds = device_state(id);
+  unsigned char *sp = nullptr;
+  if ((rc = noalloc_scratch(ds, &sp))) return rc;
+  RunOpts opt;
+  opt.no_alloc = true;
+  opt.fallback_status = (int *)sp;
+  return run_ops(ops, sp + kNoAllocStatusBytes, kNoAllocScratchBytes - kNoAllocStatusBytes, (hipStream_t)stream,
+                 opt);
+}
+
+size_t fg_noalloc_scratch_bytes(void) { return kNoAllocScratchBytes - kNoAllocStatusBytes; }
 
 static int to_specs(const fg_instance *inst, int n, std::vector<OpSpec> &ops) {
   if (n < 0 || (n > 0 && !inst)) return fail(FG_EINVAL, "bad instance array");
@@ -2095,7 +2167,20 @@ int fg_status(void *stream) {
   if (e != hipSuccess) return fail(FG_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
   e = hipStreamSynchronize((hipStream_t)stream);
   if (e != hipSuccess) return fail(FG_EHIP, std::string("hipStreamSynchronize: ") + hipGetErrorString(e));
-  return status_error(take_status(device_state(id)), "status");
+  DeviceState &ds = device_state(id);
+  int st = take_status(ds);
+  if (ds.scratch) {  // the no-allocation entry point's word, used before a pinned one existed
+    int w = 0;
+    e = hipMemcpy(&w, ds.scratch, sizeof w, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return fail(FG_EHIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+    if (w != 0) {
+      const int zero = 0;
+      e = hipMemcpy(ds.scratch, &zero, sizeof zero, hipMemcpyHostToDevice);
+      if (e != hipSuccess) return fail(FG_EHIP, std::string("hipMemcpy: ") + hipGetErrorString(e));
+      if (st == FG_OK) st = w;
+    }
+  }
+  return status_error(st, "status");
 }
 
 int fg_pending_status(int device) {

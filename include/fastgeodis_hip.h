@@ -130,6 +130,25 @@ int fg_gsf3d_f32(const float *image, const float *mask, float *out, int C,
                  const float spacing[3], float v, float lambda, int iterations,
                  void *workspace, size_t workspace_bytes, void *stream);
 
+/* ---- no-allocation variant of fg_geodesic3d_f32.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as fg_geodesic3d_f32 without the workspace: the
+ * scratch (halo exchange, transposed image and distance) is a static device
+ * array of fg_noalloc_scratch_bytes() bytes (256 MiB less 256 B, about 33 M
+ * voxels at C = 1) that each device holds from the library's first use on it
+ * until the process ends; larger volumes return FG_EWORKSPACE.  It calls no
+ * hipMalloc, hipFree, host-pinned allocation or event creation.  All calls of
+ * this entry point on one device share that array: calls that overlap on one
+ * device (other streams, other host threads) race and must be serialised by
+ * the caller.  Device errors are reported like the other entry points'; while
+ * no other entry point has run on the device, only fg_status reports them. */
+int fg_geodesic3d_f32_noalloc(const float *image, const float *mask,
+                              float *out, int C, int Cm, int D, int H, int W,
+                              const float spacing[3], float v, float l_grad,
+                              float l_eucl, int iterations, void *stream);
+size_t fg_noalloc_scratch_bytes(void);
+
 /* ---- batched form (no counterpart in the reference, which accepts batch 1
  * only, common.h:89-95): many instances of uneven sizes, 2D (D = 1) and 3D
  * mixed, in the same persistent launches.  Each instance's result equals the
