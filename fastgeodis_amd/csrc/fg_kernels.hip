@@ -88,6 +88,22 @@ struct SweepDesc {
   long long halo0;  // first halo entry of this descriptor
   float *dxo;       // transposed-output kernels: output array, element (a, b, e) at
   long long sXe, sXa;  //   dxo[e * sXe + a * sXa + b] (b contiguous)
+  // First sweep of a transform: the input distance is the initialisation,
+  // computed from the mask as it is loaded (init_mode 1: fl(v*m), 2:
+  // fl(v*fl(1-m))), and the sweep's untouched first slice is written too.
+  const float *init_src;  // mask channel 0 in this sweep's layout (init_mode != 0)
+  float init_v;
+  int init_mode;
+};
+
+// Transposes, optionally combining several sources elementwise (the epilogues
+// of signed and GSF transforms, applied while their distances leave the W
+// pass layout).
+enum XposeKind : int {
+  XK_COPY = 0,       // dst = src
+  XK_SIGNED = 1,     // dst = src - src2                      (D(M) - D(1-M))
+  XK_GSF_MID = 2,    // ds = src - src2; dst..dst4 = the inits of the four second-stage transforms
+  XK_GSF_FINAL = 3,  // dst = -(src2 - src3) + (src4 - src)
 };
 
 struct XposeDesc {  // out[c][w][h][d] = in[c][d][h][w]
@@ -96,6 +112,10 @@ struct XposeDesc {  // out[c][w][h][d] = in[c][d][h][w]
   int C, D, H, W;
   int tiles_dw;  // tiles per (c, h) plane
   int tile0;
+  int kind;                          // XposeKind (C == 1 unless XK_COPY)
+  const float *src2, *src3, *src4;   // further sources, same layout as src
+  float *dst2, *dst3, *dst4;         // further outputs (XK_GSF_MID), same layout as dst
+  float v, theta;
 };
 
 // ---------------------------------------------------------------------------
@@ -347,8 +367,16 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   // waves' rows R-1 and 0: they arrive through LDS when the slab is consumed
   // (see step), so only rows 0..R-1 are loaded.  The initial front and the
   // one-wave kernel load all rows.
+  const int init_mode = Dd.init_mode;
+  const float init_v = Dd.init_v;
+  // the initialisation from the loaded mask value (uniform branch)
+  auto init_of = [&](float m) -> float {
+    if (init_mode == 1) return init_v * m;
+    if (init_mode == 2) return init_v * (1.0f - m);
+    return m;
+  };
   auto load_rows = [&](Slab<R, CT> &S, int a, int r0, int r1) {
-    const __amdgpu_buffer_rsrc_t rd = slice_rsrc(Dd.d, a);
+    const __amdgpu_buffer_rsrc_t rd = slice_rsrc(init_mode ? Dd.init_src : Dd.d, a);
     __amdgpu_buffer_rsrc_t ri[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) ri[c] = slice_rsrc(Dd.img + c * cs, a);
@@ -385,7 +413,16 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   Slab<R, CT> S[P];
   load_rows(S[P - 1], a_first - dir, 0, R + 2);  // front slice: not modified by this sweep
 #pragma unroll
-  for (int r = 0; r < R; ++r) Fd[r] = pvalid[r] ? S[P - 1].d[r] : kInf;
+  for (int r = 0; r < R; ++r) Fd[r] = pvalid[r] ? init_of(S[P - 1].d[r]) : kInf;
+  if (init_mode) {
+    // the output's copy of the untouched first slice: its initialisation
+    const __amdgpu_buffer_rsrc_t rs = slice_rsrc(Dd.d, a_first - dir);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const unsigned off = (row_own[r] && lane_own) ? rofs[r + 1] : 0xFFFFFFFFu;
+      __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(Fd[r]), rs, (int)off, 0, 0);
+    }
+  }
   if constexpr (XST) {  // stage the first slice as step -1 (ring slot P - 1)
 #pragma unroll
     for (int r = 0; r < R; ++r) xtile[P - 1][w * R + r][lane] = Fd[r];
@@ -515,7 +552,7 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
         Iq[2][c] = Fr.I[c][r + 2];
         In[c] = Sl.I[c][r + 1];
       }
-      const float v = relax<CT, LAMB1, FAST>(dq, Iq, In, Sl.d[r], e1, lg);
+      const float v = relax<CT, LAMB1, FAST>(dq, Iq, In, init_of(Sl.d[r]), e1, lg);
       nd[r] = pvalid[r] ? v : kInf;
     }
     // own outputs: raw buffer stores, non-own lanes and rows get an
@@ -614,6 +651,30 @@ __global__ void slice_kernel(SliceDesc D, int a, int aq) {
 
 constexpr int kXT = 64;
 
+// Value of the combined sources at element k of the source layout.
+__device__ __forceinline__ float xsrc(const XposeDesc &X, long long k) {
+  const float a = X.src[k];
+  if (X.kind == XK_SIGNED || X.kind == XK_GSF_MID) return a - X.src2[k];
+  if (X.kind == XK_GSF_FINAL) return -(X.src2[k] - X.src3[k]) + (X.src4[k] - a);
+  return a;
+}
+
+// Writes the (combined) value x to element k of the destination layout(s).
+__device__ __forceinline__ void xdst(const XposeDesc &X, long long k, float x) {
+  if (X.kind == XK_GSF_MID) {
+    // x = Ds = D(M) - D(1-M); Md = Ds > theta, Me = Ds > -theta
+    const float md = (x > X.theta) ? 1.0f : 0.0f;
+    const float me = (x > -X.theta) ? 1.0f : 0.0f;
+    const float m1 = 1.0f - md;  // mask of signed(image, 1 - Md)
+    X.dst2[k] = X.v * m1;        // D(1-Md) init
+    X.dst3[k] = X.v * (1.0f - m1);  // D(1-(1-Md)) init
+    X.dst4[k] = X.v * me;        // D(Me) init
+    X.dst[k] = X.v * (1.0f - me);   // D(1-Me) init
+  } else {
+    X.dst[k] = x;
+  }
+}
+
 __global__ __launch_bounds__(256) void xpose_kernel(const XposeDesc *__restrict__ descs, int ndesc, XposeDesc one) {
   __shared__ float tile[kXT][kXT + 1];
   const int t = blockIdx.x;
@@ -636,12 +697,14 @@ __global__ __launch_bounds__(256) void xpose_kernel(const XposeDesc *__restrict_
   const int d0 = td * kXT, w0 = tw * kXT;
   const long long HW = (long long)X.H * X.W, HD = (long long)X.H * X.D;
   const long long cn = (long long)X.D * HW;
-  const float *src = X.src + c * cn + (long long)h * X.W;
-  float *dst = X.dst + c * cn + (long long)h * X.D;
-  if ((X.W & 3) == 0 && (X.D & 3) == 0 && d0 + kXT <= X.D && w0 + kXT <= X.W &&
+  const long long sbase = c * cn + (long long)h * X.W;  // element offsets of this (c, h) plane
+  const long long dbase = c * cn + (long long)h * X.D;
+  if (X.kind == XK_COPY && (X.W & 3) == 0 && (X.D & 3) == 0 && d0 + kXT <= X.D && w0 + kXT <= X.W &&
       (((uintptr_t)X.src | (uintptr_t)X.dst) & 15) == 0) {
     // full tile, 16-byte rows: each thread moves float4s (16 threads per
     // 256-byte row segment) and the LDS tile keeps its padded scalar layout
+    const float *src = X.src + sbase;
+    float *dst = X.dst + dbase;
     const int q = threadIdx.x & 15, r0 = threadIdx.x >> 4;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -668,12 +731,12 @@ __global__ __launch_bounds__(256) void xpose_kernel(const XposeDesc *__restrict_
   const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
   for (int r = ty; r < kXT; r += 4) {
     const int d = d0 + r, w = w0 + tx;
-    if (d < X.D && w < X.W) tile[r][tx] = src[(long long)d * HW + w];
+    if (d < X.D && w < X.W) tile[r][tx] = xsrc(X, sbase + (long long)d * HW + w);
   }
   __syncthreads();
   for (int r = ty; r < kXT; r += 4) {
     const int w = w0 + r, d = d0 + tx;
-    if (d < X.D && w < X.W) dst[(long long)w * HD + d] = tile[tx][r];
+    if (d < X.D && w < X.W) xdst(X, dbase + (long long)w * HD + d, tile[tx][r]);
   }
 }
 
@@ -688,12 +751,14 @@ enum EwKind : int {
   EW_GSF_MID = 3,        // from (out0, a): thresholds -> inits of the 4 stage-2 transforms
   EW_GSF_FINAL = 4,      // out0 = -(a - b) + (c2 - out0); channels >= 1 in closed form
   EW_FILL_U64 = 5,       // zero a range of 64-bit words
+  EW_CHANNELS = 6,       // mask channels >= 1 (elements [n0, n0*Cm)): the op's closed form of the mask
 };
 
 struct EwDesc {
   int kind;
   long long n0;   // voxels per channel
   int Cm;
+  int op;         // EW_CHANNELS: FG_OP_*
   const float *mask;
   float *out;
   float *a, *b, *c2;
@@ -792,6 +857,23 @@ __device__ __forceinline__ void ew_one(const EwDesc &E, long long i) {
       case EW_FILL_U64:
         E.u64[i] = 0ull;
         break;
+      case EW_CHANNELS: {
+        // element n0 + i: an untransformed channel (reference: channels >= 1
+        // keep their initialisation through every pass)
+        const long long k = E.n0 + i;
+        const float m = E.mask[k];
+        if (E.op == FG_OP_UNSIGNED) {
+          E.out[k] = v * m;
+        } else if (E.op == FG_OP_SIGNED) {
+          E.out[k] = signed_closed(v, m);
+        } else {
+          const float ds = signed_closed(v, m);
+          const float md = (ds > E.theta) ? 1.0f : 0.0f;
+          const float me = (ds > -E.theta) ? 1.0f : 0.0f;
+          E.out[k] = -signed_closed(v, 1.0f - md) + signed_closed(v, me);
+        }
+        break;
+      }
       default:
         break;
     }
@@ -865,7 +947,23 @@ struct Xform {
   bool lamb1, fast;
   int iters;
   bool xst;  // this iteration's H-backward sweep writes dT directly (no transpose before the W pass)
+  // The first sweep initialises the distance from the mask (init_mode 1: v*m,
+  // 2: v*(1-m)) instead of reading it; set only when that sweep is tiled.
+  int init_mode = 0;
+  const float *init_src = nullptr;
+  float init_v = 0.f;
+  // Transpose back after the last iteration: the first transform of a group
+  // (signed pair, GSF stages) combines the group's distances (fin_kind) into
+  // fin_dst; the other members (fin_member) contribute their dT only.
+  int fin_kind = XK_COPY;
+  bool fin_member = false;
+  const float *fin_src[3] = {nullptr, nullptr, nullptr};
+  float *fin_dst[4] = {nullptr, nullptr, nullptr, nullptr};
+  float fin_v = 0.f, fin_theta = 0.f;
 };
+
+// The pass of a transform's first sweep: rows (1) in 2D, D (0) in 3D.
+int first_pass(const Xform &x) { return x.is2d ? 1 : 0; }
 
 // Tiled plan for one sweep plane.
 struct Plan {
@@ -1249,8 +1347,15 @@ struct Runner {
   unsigned tag_base = 0;
   Recorder rec;
 
+  int cur_it = 0;  // iteration being recorded
+
   void fill_desc(const Xform &x, int pass, int dir, SweepDesc &s, bool xst = false) {
     std::memset(&s, 0, sizeof s);
+    if (x.init_mode && cur_it == 0 && pass == first_pass(x) && dir > 0) {
+      s.init_src = x.init_src;
+      s.init_v = x.init_v;
+      s.init_mode = x.init_mode;
+    }
     const long long N = (long long)x.D * x.H * x.W;
     s.C = x.C;
     s.cs = N;
@@ -1579,6 +1684,7 @@ struct Runner {
     int max_it = 0;
     for (auto &x : xs) max_it = std::max(max_it, x.iters);
     for (int it = 0; it < max_it; ++it) {
+      cur_it = it;
       std::vector<Xform *> active;
       for (auto &x : xs)
         if (x.iters > it) active.push_back(&x);
@@ -1591,12 +1697,49 @@ struct Runner {
         int D, H, W;
         native_dims(*x, D, H, W);
         if (!x->xst) to.push_back(xd(x->d, x->dT, 1, D, H, W));
-        back.push_back(xd(x->dT, x->d, 1, W, H, D));
+        const bool last = it == x->iters - 1;
+        if (last && x->fin_member) continue;  // part of its group's combined transpose
+        XposeDesc b = xd(x->dT, x->d, 1, W, H, D);
+        if (last && x->fin_kind != XK_COPY) {
+          b.kind = x->fin_kind;
+          b.src2 = x->fin_src[0];
+          b.src3 = x->fin_src[1];
+          b.src4 = x->fin_src[2];
+          b.dst = x->fin_dst[0];
+          b.dst2 = x->fin_dst[1];
+          b.dst3 = x->fin_dst[2];
+          b.dst4 = x->fin_dst[3];
+          b.v = x->fin_v;
+          b.theta = x->fin_theta;
+        }
+        back.push_back(b);
       }
       record_xpose(to);
       if ((rc = sweep_pass(active, 2))) return rc;
       record_xpose(back);
     }
+    cur_it = 0;
+    return FG_OK;
+  }
+
+  // Whether the first sweep of x runs on the tiled engine whatever halo depth
+  // its pass is planned with (the plane fits the device's co-resident
+  // workgroups even at the shallowest halo, which has the most tiles).
+  int first_sweep_tiled(const Xform &x, bool &tiled) {
+    tiled = false;
+    const int pass = first_pass(x);
+    if (x.iters <= 0 || !tiled_eligible(x, pass)) return FG_OK;
+    int nA, nB, nE;
+    pass_plane(x, pass, nA, nB, nE);
+    const Plan p = plan_fixed(nB, nE, K > 0 ? K : 1, x.C);
+    if (!p.ok) return FG_OK;
+    const void *kern = pick_kernel(p.NW, p.R, p.P, x.C, x.lamb1, x.fast, false);
+    if (!kern) return FG_OK;
+    long long cap = 0;
+    const int rc = coresident_groups(kern, p.NW * 64, dev, cap);
+    if (rc) return rc;
+    if (max_groups > 0) cap = std::min<long long>(cap, max_groups);
+    tiled = (long long)p.nTB * p.nTE <= cap;
     return FG_OK;
   }
 
@@ -1856,30 +1999,31 @@ int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream, 
     run.record_xpose(xs);
   }
 
-  // stage 1: initialise and transform
-  std::vector<EwDesc> init;
+  // Stage 1: initialise and transform.  With iterations, a transform's first
+  // sweep computes its initialisation from the mask (when that sweep is
+  // tiled), mask channels >= 1 get their closed form in one elementwise pass,
+  // and each op's epilogue is applied by its last transpose back (signed:
+  // D(M) - D(1-M); GSF: the thresholds that initialise stage 2).  Without
+  // iterations the elementwise kernels do all of it.
+  std::vector<EwDesc> init, chans;
   std::vector<Xform> xs;
+  std::vector<char> fused(ops.size(), 0);
   for (size_t i = 0; i < ops.size(); ++i) {
     const OpSpec &o = ops[i];
     const long long N = vox(o);
+    fused[i] = o.iters > 0;
     EwDesc e{};
     e.n0 = N;
     e.Cm = o.Cm;
+    e.op = o.op;
     e.mask = o.mask;
     e.out = o.out;
     e.v = o.v;
     e.theta = o.theta;
-    e.nelem = N * o.Cm;
     float lg = o.lg, le = o.le;
     if (o.op == FG_OP_GSF) le = 1 - o.lg;  // reference fastgeodis.cpp:321: 1 - lambda in float
-    if (o.op == FG_OP_UNSIGNED) {
-      e.kind = EW_INIT_UNSIGNED;
-    } else {
-      e.kind = EW_INIT_SIGNED;
-      e.a = bufs[i].dist[1];
-    }
-    init.push_back(e);
     const int nd = (o.op == FG_OP_UNSIGNED) ? 1 : 2;
+    const size_t j0 = xs.size();
     for (int k = 0; k < nd; ++k) {
       Xform x{};
       make_xform(o, lg, le, x);
@@ -1889,11 +2033,49 @@ int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream, 
       x.dT = bufs[i].dT[k];
       xs.push_back(x);
     }
+    EwDesc ie = e;
+    ie.kind = (o.op == FG_OP_UNSIGNED) ? EW_INIT_UNSIGNED : EW_INIT_SIGNED;
+    ie.a = bufs[i].dist[1];
+    if (!fused[i]) {
+      ie.nelem = N * o.Cm;  // every channel
+      init.push_back(ie);
+      continue;
+    }
+    if (o.Cm > 1) {
+      EwDesc c = e;
+      c.kind = EW_CHANNELS;
+      c.nelem = N * (o.Cm - 1);
+      chans.push_back(c);
+    }
+    bool tiled = false;
+    if ((rc = run.first_sweep_tiled(xs[j0], tiled))) return rc;
+    if (tiled) {
+      for (int k = 0; k < nd; ++k) {
+        xs[j0 + k].init_mode = (k == 0) ? 1 : 2;
+        xs[j0 + k].init_src = o.mask;
+        xs[j0 + k].init_v = o.v;
+      }
+    } else {
+      ie.nelem = N;  // channel 0
+      init.push_back(ie);
+    }
+    if (o.op != FG_OP_UNSIGNED) {
+      Xform &x0 = xs[j0];
+      x0.fin_kind = (o.op == FG_OP_SIGNED) ? XK_SIGNED : XK_GSF_MID;
+      x0.fin_src[0] = xs[j0 + 1].dT;
+      x0.fin_dst[0] = o.out;
+      if (o.op == FG_OP_GSF)
+        for (int k = 1; k < 4; ++k) x0.fin_dst[k] = bufs[i].dist[k];
+      x0.fin_v = o.v;
+      x0.fin_theta = o.theta;
+      xs[j0 + 1].fin_member = true;
+    }
   }
   run.record_ew(init);
+  run.record_ew(chans);
   if ((rc = run.record_run(xs))) return rc;
 
-  // stage 2 (GSF): thresholds, four more transforms
+  // Stage 2 (GSF): four more transforms from the thresholds.
   std::vector<EwDesc> mid, fin;
   std::vector<Xform> xs2;
   for (size_t i = 0; i < ops.size(); ++i) {
@@ -1902,29 +2084,33 @@ int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream, 
     EwDesc e{};
     e.n0 = N;
     e.Cm = o.Cm;
+    e.op = o.op;
     e.mask = o.mask;
     e.out = o.out;
     e.v = o.v;
     e.theta = o.theta;
-    if (o.op == FG_OP_SIGNED) {
+    if (o.op == FG_OP_SIGNED && !fused[i]) {
       e.kind = EW_FINAL_SIGNED;
       e.a = bufs[i].dist[1];
       e.nelem = N * o.Cm;
       fin.push_back(e);
     } else if (o.op == FG_OP_GSF) {
-      EwDesc m = e;
-      m.kind = EW_GSF_MID;
-      m.a = bufs[i].dist[1];
-      m.b = bufs[i].dist[2];
-      m.c2 = bufs[i].dist[3];
-      m.nelem = N;
-      mid.push_back(m);
-      EwDesc f = m;
-      f.kind = EW_GSF_FINAL;
-      f.nelem = N * o.Cm;
-      fin.push_back(f);
+      if (!fused[i]) {
+        EwDesc m = e;
+        m.kind = EW_GSF_MID;
+        m.a = bufs[i].dist[1];
+        m.b = bufs[i].dist[2];
+        m.c2 = bufs[i].dist[3];
+        m.nelem = N;
+        mid.push_back(m);
+        EwDesc f = m;
+        f.kind = EW_GSF_FINAL;
+        f.nelem = N * o.Cm;
+        fin.push_back(f);
+      }
       const float le = 1 - o.lg;
       // dist[0] = out: D(1-Me); dist[1]: D(1-Md); dist[2]: D(Md); dist[3]: D(Me)
+      const size_t j0 = xs2.size();
       for (int k = 0; k < 4; ++k) {
         Xform x{};
         make_xform(o, o.lg, le, x);
@@ -1934,12 +2120,19 @@ int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream, 
         x.dT = bufs[i].dT[k];
         xs2.push_back(x);
       }
+      if (fused[i]) {
+        Xform &x0 = xs2[j0];
+        x0.fin_kind = XK_GSF_FINAL;
+        for (int k = 1; k < 4; ++k) {
+          x0.fin_src[k - 1] = xs2[j0 + k].dT;
+          xs2[j0 + k].fin_member = true;
+        }
+        x0.fin_dst[0] = o.out;
+      }
     }
   }
-  if (!mid.empty()) {
-    run.record_ew(mid);
-    if ((rc = run.record_run(xs2))) return rc;
-  }
+  run.record_ew(mid);
+  if (!xs2.empty() && (rc = run.record_run(xs2))) return rc;
   run.record_ew(fin);
   return run.replay(stream, L.blob, L.blob_cap, L.halo, status, opt.no_alloc);
 }
