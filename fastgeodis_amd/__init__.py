@@ -205,6 +205,14 @@ def get_tuning():
     return {"engine": e, "halo": h, "max_groups": g, "shape": s}
 
 
+def set_graphs(on: bool = True) -> None:
+    """Graph replay on this host thread (default on): a call whose sequence of
+    kernel launches repeats one seen before runs as one launch of a cached HIP
+    graph updated with the call's arguments.  Results are identical either
+    way; off issues every launch directly."""
+    FastGeodisCpp.set_graphs(bool(on))
+
+
 def check_status(device: Optional[int] = None) -> None:
     """Synchronises the device's current stream and raises RuntimeError if a
     launch on it reported an error (an expired bounded wait, whose call then
@@ -226,5 +234,5 @@ def native_library_paths() -> List[str]:
 __all__ = [
     "generalised_geodesic2d", "generalised_geodesic3d",
     "signed_generalised_geodesic2d", "signed_generalised_geodesic3d",
-    "GSF2d", "GSF3d", "Instance", "geodesic_batch", "set_tuning", "get_tuning", "check_status",
+    "GSF2d", "GSF3d", "Instance", "geodesic_batch", "set_tuning", "get_tuning", "check_status", "set_graphs",
 ]

@@ -328,6 +328,8 @@ void inject_status(int64_t device, int64_t code) { raise_for(fg_inject_status((i
 
 void set_wait_limit(int64_t spins) { raise_for(fg_set_wait_limit((int)spins)); }
 
+void set_graphs(bool on) { raise_for(fg_set_graphs(on ? 1 : 0)); }
+
 int64_t workspace_bytes(int64_t op, int64_t C, int64_t D, int64_t H, int64_t W) {
   return (int64_t)fg_workspace_bytes((int)op, (int)C, (int)D, (int)H, (int)W);
 }
@@ -352,5 +354,6 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("pending_status", &pending_status, "Read and clear a device's pending status code without synchronising");
   m.def("inject_status", &inject_status, "Set a device's pending status from the host (error-path tests)");
   m.def("set_wait_limit", &set_wait_limit, "Spin limit of the bounded halo waits for this host thread (0 = default)");
-  m.def("release", []() { fg_release(); }, "Free the library's pinned memory and events (process exit)");
+  m.def("release", []() { fg_release(); }, "Free the library's pinned memory, events and graphs (process exit)");
+  m.def("set_graphs", &set_graphs, "Graph replay of repeated launch sequences on this host thread (default on)");
 }

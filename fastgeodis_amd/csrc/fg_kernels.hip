@@ -45,6 +45,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <cmath>
 #include <cstdint>
@@ -53,6 +54,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "fastgeodis_hip.h"
@@ -236,7 +238,9 @@ struct Slab {  // one prefetched slice: image rows -1..R, distance rows 0..R-1
 // with B contiguous (the H pass writes the (W, H, D) layout of the W pass),
 // staging each slice's window through LDS so each store covers a run of
 // consecutive B.
-template <int NW, int R, int P, int CT, bool LAMB1, bool FAST, bool XST = false>
+// INIT: the first sweep of a transform, whose input distance is computed from
+// the mask as it is loaded (SweepDesc::init_mode).
+template <int NW, int R, int P, int CT, bool LAMB1, bool FAST, bool XST = false, bool INIT = false>
 __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     const SweepDesc *__restrict__ descs, int ndesc, SweepDesc one, int K, unsigned tag_base,
     unsigned long long *__restrict__ halo_base, int *__restrict__ status, int spin_limit) {
@@ -367,16 +371,20 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   // waves' rows R-1 and 0: they arrive through LDS when the slab is consumed
   // (see step), so only rows 0..R-1 are loaded.  The initial front and the
   // one-wave kernel load all rows.
-  const int init_mode = Dd.init_mode;
+  // the initialisation from the loaded mask value: fl(v*m) (mode 1) or
+  // fl(v*fl(1-m)) (mode 2), by a select rather than a branch
+  const bool init_inv = Dd.init_mode == 2;
   const float init_v = Dd.init_v;
-  // the initialisation from the loaded mask value (uniform branch)
   auto init_of = [&](float m) -> float {
-    if (init_mode == 1) return init_v * m;
-    if (init_mode == 2) return init_v * (1.0f - m);
-    return m;
+    if constexpr (INIT) {
+      const float t = 1.0f - m;
+      return init_v * (init_inv ? t : m);
+    } else {
+      return m;
+    }
   };
   auto load_rows = [&](Slab<R, CT> &S, int a, int r0, int r1) {
-    const __amdgpu_buffer_rsrc_t rd = slice_rsrc(init_mode ? Dd.init_src : Dd.d, a);
+    const __amdgpu_buffer_rsrc_t rd = slice_rsrc(INIT ? Dd.init_src : Dd.d, a);
     __amdgpu_buffer_rsrc_t ri[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) ri[c] = slice_rsrc(Dd.img + c * cs, a);
@@ -414,7 +422,7 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   load_rows(S[P - 1], a_first - dir, 0, R + 2);  // front slice: not modified by this sweep
 #pragma unroll
   for (int r = 0; r < R; ++r) Fd[r] = pvalid[r] ? init_of(S[P - 1].d[r]) : kInf;
-  if (init_mode) {
+  if constexpr (INIT) {
     // the output's copy of the untouched first slice: its initialisation
     const __amdgpu_buffer_rsrc_t rs = slice_rsrc(Dd.d, a_first - dir);
 #pragma unroll
@@ -1050,29 +1058,34 @@ void pass_plane(const Xform &x, int pass, int &nA, int &nB, int &nE) {
   else { nA = x.W; nB = x.H; nE = x.D; }
 }
 
-template <int NW, int R, int P, int CT, bool L1, bool FA, bool XS = false>
+template <int NW, int R, int P, int CT, bool L1, bool FA, bool XS = false, bool IN = false>
 const void *kernel_ptr() {
-  return reinterpret_cast<const void *>(&sweep_tiled_kernel<NW, R, P, CT, L1, FA, XS>);
+  return reinterpret_cast<const void *>(&sweep_tiled_kernel<NW, R, P, CT, L1, FA, XS, IN>);
 }
 
 // lamb1 implies fast (l_grad == 1 and l_eucl*LD == +0 are non-negative).
 // Only the channel counts a (shape, ring) pair serves are instantiated.
-template <int NW, int R, int P, int CT, bool XS = false>
+template <int NW, int R, int P, int CT, bool XS = false, bool IN = false>
 const void *pick_variant(bool lamb1, bool fast) {
-  if (lamb1) return kernel_ptr<NW, R, P, CT, true, true, XS>();
-  return fast ? kernel_ptr<NW, R, P, CT, false, true, XS>() : kernel_ptr<NW, R, P, CT, false, false, XS>();
+  if (lamb1) return kernel_ptr<NW, R, P, CT, true, true, XS, IN>();
+  return fast ? kernel_ptr<NW, R, P, CT, false, true, XS, IN>() : kernel_ptr<NW, R, P, CT, false, false, XS, IN>();
+}
+
+template <int NW, int R, int P, int CT>
+const void *pick_variant_init(bool lamb1, bool fast, bool init) {
+  return init ? pick_variant<NW, R, P, CT, false, true>(lamb1, fast) : pick_variant<NW, R, P, CT>(lamb1, fast);
 }
 
 template <int NW, int R, int P, int CLO, int CHI>
-const void *pick_kernel_nr(int C, bool lamb1, bool fast) {
+const void *pick_kernel_nr(int C, bool lamb1, bool fast, bool init) {
   if constexpr (CLO <= 1 && 1 <= CHI)
-    if (C == 1) return pick_variant<NW, R, P, 1>(lamb1, fast);
+    if (C == 1) return pick_variant_init<NW, R, P, 1>(lamb1, fast, init);
   if constexpr (CLO <= 2 && 2 <= CHI)
-    if (C == 2) return pick_variant<NW, R, P, 2>(lamb1, fast);
+    if (C == 2) return pick_variant_init<NW, R, P, 2>(lamb1, fast, init);
   if constexpr (CLO <= 3 && 3 <= CHI)
-    if (C == 3) return pick_variant<NW, R, P, 3>(lamb1, fast);
+    if (C == 3) return pick_variant_init<NW, R, P, 3>(lamb1, fast, init);
   if constexpr (CLO <= 4 && 4 <= CHI)
-    if (C == 4) return pick_variant<NW, R, P, 4>(lamb1, fast);
+    if (C == 4) return pick_variant_init<NW, R, P, 4>(lamb1, fast, init);
   return nullptr;
 }
 
@@ -1080,7 +1093,8 @@ const void *pick_kernel_nr(int C, bool lamb1, bool fast) {
 // Transposed-output kernels exist for the two 5-slice-ring 3D shapes: 8 x 4
 // with 1-2 channels, 16 x 2 with 1 channel (2 channels would exceed the LDS).
 constexpr int kXstMaxC = 2;
-const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast, bool xst = false) {
+const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast, bool xst = false, bool init = false) {
+  if (xst && init) return nullptr;
   if (xst) {
     if (NW == 8 && R == 4 && P == 5 && C == 1) return pick_variant<8, 4, 5, 1, true>(lamb1, fast);
     if (NW == 8 && R == 4 && P == 5 && C == 2) return pick_variant<8, 4, 5, 2, true>(lamb1, fast);
@@ -1088,7 +1102,7 @@ const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast, bool
     return nullptr;
   }
 #define FG_S(nw, r, p, clo, chi) \
-  if (NW == nw && R == r && P == p && C >= clo && C <= chi) return pick_kernel_nr<nw, r, p, clo, chi>(C, lamb1, fast);
+  if (NW == nw && R == r && P == p && C >= clo && C <= chi) return pick_kernel_nr<nw, r, p, clo, chi>(C, lamb1, fast, init);
   FG_S(1, 1, 8, 1, 4)
   FG_S(4, 8, 2, 1, 4)
   FG_S(8, 4, 3, 1, 2)
@@ -1332,6 +1346,36 @@ int coresident_groups(const void *kern, int threads, const Device &dev, long lon
   return FG_OK;
 }
 
+// Executable graphs of repeated launch sequences.  A replay whose sequence of
+// launches (kernel, grid and block of each, in order) repeats on a host
+// thread and device runs as one hipGraphLaunch of a cached graph whose kernel
+// nodes (a chain) get this call's arguments; the first occurrence of a
+// sequence launches directly, the second builds the graph.  Graphs hold
+// kernel nodes only (no memset or copy nodes).
+thread_local bool g_graph_on = true;
+
+struct GraphEntry {
+  std::thread::id thread;
+  uint64_t key = 0;
+  int dev = -1;
+  hipGraph_t graph{};
+  hipGraphExec_t exec{};
+  std::vector<hipGraphNode_t> nodes;
+  uint64_t last_use = 0;
+};
+std::mutex g_graph_m;
+std::vector<std::unique_ptr<GraphEntry>> g_graph_list;  // every thread's entries
+thread_local std::vector<std::pair<uint64_t, int>> g_graph_seen;  // sequences seen once (key, device)
+thread_local uint64_t g_graph_clock = 0;
+constexpr int kGraphsPerThread = 8;
+
+void destroy_graph(GraphEntry &e) {
+  if (e.exec) (void)hipGraphExecDestroy(e.exec);
+  if (e.graph) (void)hipGraphDestroy(e.graph);
+  e.exec = nullptr;
+  e.graph = nullptr;
+}
+
 struct SweepJob {
   Xform *x;
   int pass;
@@ -1424,7 +1468,7 @@ struct Runner {
   }
 
   int record_tiled(std::vector<SweepJob> &jobs, int dir, int NW, int R, int P, int C, bool lamb1, bool fast,
-                   bool xs = false) {
+                   bool xs = false, bool in = false) {
     std::vector<SweepDesc> descs;
     long long halo_used = 0;
     int tiles = 0, max_blocks = 0, kk = -1;
@@ -1454,7 +1498,7 @@ struct Runner {
     }
     Cmd c;
     c.type = Cmd::TILED;
-    c.kern = pick_kernel(NW, R, P, C, lamb1, fast, xs);
+    c.kern = pick_kernel(NW, R, P, C, lamb1, fast, xs, in);
     if (!c.kern) return fail(FG_EINVAL, "no tiled kernel for this shape");
     c.grid = tiles;
     c.block = NW * 64;
@@ -1608,14 +1652,16 @@ struct Runner {
   int sweep_pass(std::vector<Xform *> &active, int pass) {
     const std::vector<int> ks = pass_halos(active, pass);
     for (int dir : {+1, -1}) {
-      struct G { int NW, R, P, K, C; bool l1, fa, xs; std::vector<SweepJob> jobs; };
+      struct G { int NW, R, P, K, C; bool l1, fa, xs, in; std::vector<SweepJob> jobs; };
       std::vector<G> groups;
       for (size_t i = 0; i < active.size(); ++i) {
         Xform *x = active[i];
         int nA, nB, nE;
         pass_plane(*x, pass, nA, nB, nE);
         if (nA <= 1) continue;
+        const bool in = x->init_mode && cur_it == 0 && pass == first_pass(*x) && dir > 0;
         if (ks[i] == 0) {
+          if (in) return fail(FG_EINVAL, "internal: an initialising sweep was planned on the per-slice engine");
           record_per_slice(*x, pass, dir);
           continue;
         }
@@ -1624,15 +1670,15 @@ struct Runner {
         bool found = false;
         for (auto &g : groups)
           if (g.NW == p.NW && g.R == p.R && g.P == p.P && g.K == p.K && g.C == x->C && g.l1 == x->lamb1 &&
-              g.fa == x->fast && g.xs == xs) {
+              g.fa == x->fast && g.xs == xs && g.in == in) {
             g.jobs.push_back({x, pass, p.K});
             found = true;
             break;
           }
-        if (!found) groups.push_back({p.NW, p.R, p.P, p.K, x->C, x->lamb1, x->fast, xs, {{x, pass, p.K}}});
+        if (!found) groups.push_back({p.NW, p.R, p.P, p.K, x->C, x->lamb1, x->fast, xs, in, {{x, pass, p.K}}});
       }
       for (auto &g : groups) {
-        int rc = record_group(g.jobs, dir, g.NW, g.R, g.P, g.C, g.l1, g.fa, g.xs);
+        int rc = record_group(g.jobs, dir, g.NW, g.R, g.P, g.C, g.l1, g.fa, g.xs, g.in);
         if (rc) return rc;
       }
     }
@@ -1641,8 +1687,8 @@ struct Runner {
 
   // A group is split into launches whose tiles can all be co-resident.
   int record_group(std::vector<SweepJob> &jobs, int dir, int NW, int R, int P, int C, bool l1, bool fa,
-                   bool xs = false) {
-    const void *kern = pick_kernel(NW, R, P, C, l1, fa, xs);
+                   bool xs = false, bool in = false) {
+    const void *kern = pick_kernel(NW, R, P, C, l1, fa, xs, in);
     if (!kern) return fail(FG_EINVAL, "no tiled kernel for this shape");
     long long cap = 0;
     int rc = coresident_groups(kern, NW * 64, dev, cap);
@@ -1664,19 +1710,20 @@ struct Runner {
       Plan p = plan_fixed(nB, nE, j.K, j.x->C);
       const long long nt = (long long)p.nTB * p.nTE;
       if (nt > cap) {  // one plane alone cannot be co-resident: per-slice engine
+        if (in) return fail(FG_EINVAL, "internal: an initialising sweep was planned on the per-slice engine");
         record_per_slice(*j.x, j.pass, dir);
         j.x->xst = false;  // its output stays in the native layout
         continue;
       }
       if (tiles + nt > cap) {
-        if ((rc = record_tiled(chunk, dir, NW, R, P, C, l1, fa, xs))) return rc;
+        if ((rc = record_tiled(chunk, dir, NW, R, P, C, l1, fa, xs, in))) return rc;
         chunk.clear();
         tiles = 0;
       }
       chunk.push_back(j);
       tiles += nt;
     }
-    if (!chunk.empty()) return record_tiled(chunk, dir, NW, R, P, C, l1, fa, xs);
+    if (!chunk.empty()) return record_tiled(chunk, dir, NW, R, P, C, l1, fa, xs, in);
     return FG_OK;
   }
 
@@ -1733,7 +1780,7 @@ struct Runner {
     pass_plane(x, pass, nA, nB, nE);
     const Plan p = plan_fixed(nB, nE, K > 0 ? K : 1, x.C);
     if (!p.ok) return FG_OK;
-    const void *kern = pick_kernel(p.NW, p.R, p.P, x.C, x.lamb1, x.fast, false);
+    const void *kern = pick_kernel(p.NW, p.R, p.P, x.C, x.lamb1, x.fast, false, true);
     if (!kern) return FG_OK;
     long long cap = 0;
     const int rc = coresident_groups(kern, p.NW * 64, dev, cap);
@@ -1766,42 +1813,153 @@ struct Runner {
     if (!rec.blob.empty() && (rc = stage_copy(ds, blob_dev, rec.blob.data(), rec.blob.size(), stream))) return rc;
     // the kernel expires a wait when a check finds a halo value pending after spin re-polls
     const int spin = g_wait_limit > 0 ? g_wait_limit - 1 : (1 << 21);
-    for (auto &c : rec.cmds) {
+
+    // Argument values of every launch (the arrays point into `vals`, which
+    // outlives the launches and graph updates below).
+    struct Vals {
+      const void *dp;
+      int ndesc, k, sp, a, aq;
+      unsigned tb;
+    };
+    const size_t n = rec.cmds.size();
+    std::vector<Vals> vals(n);
+    std::vector<std::array<void *, 8>> args(n);
+    std::vector<const void *> funcs(n);
+    bool slice = false;
+    for (size_t i = 0; i < n; ++i) {
+      Cmd &c = rec.cmds[i];
+      Vals &v = vals[i];
+      v.ndesc = c.ndesc;
+      auto &a = args[i];
       switch (c.type) {
-        case Cmd::TILED: {
-          const SweepDesc *dp = c.by_value ? nullptr : reinterpret_cast<const SweepDesc *>(blob_dev + c.off);
-          int ndesc = c.ndesc, k = c.K, sp = spin;
-          unsigned tb = c.tag_base;
-          void *args[] = {(void *)&dp, &ndesc, (void *)&c.sweep1, &k, &tb, (void *)&halo, (void *)&status, &sp};
-          ProfRec pr{};
-          if (g_prof_on) {
-            FG_HIP(hipEventCreate(&pr.start));
-            FG_HIP(hipEventCreate(&pr.stop));
-            FG_HIP(hipEventRecord(pr.start, stream));
-          }
-          FG_HIP(hipLaunchKernel(c.kern, dim3(c.grid), dim3(c.block), args, 0, stream));
-          if (g_prof_on) {
-            FG_HIP(hipEventRecord(pr.stop, stream));
-            pr.bytes = c.alg_bytes;
-            g_prof.push_back(pr);
-          }
+        case Cmd::TILED:
+          v.dp = c.by_value ? nullptr : blob_dev + c.off;
+          v.k = c.K;
+          v.sp = spin;
+          v.tb = c.tag_base;
+          a = {(void *)&v.dp, &v.ndesc, (void *)&c.sweep1, &v.k, &v.tb, (void *)&halo, (void *)&status, &v.sp};
+          funcs[i] = c.kern;
           break;
-        }
         case Cmd::SLICE:
-          hipLaunchKernelGGL(slice_kernel, dim3(c.grid), dim3(c.block), 0, stream, c.slice, c.a, c.aq);
+          v.a = c.a;
+          v.aq = c.aq;
+          a = {(void *)&c.slice, &v.a, &v.aq, nullptr, nullptr, nullptr, nullptr, nullptr};
+          funcs[i] = reinterpret_cast<const void *>(&slice_kernel);
+          slice = true;
           break;
-        case Cmd::XPOSE: {
-          const XposeDesc *dp = c.by_value ? nullptr : reinterpret_cast<const XposeDesc *>(blob_dev + c.off);
-          hipLaunchKernelGGL(xpose_kernel, dim3(c.grid), dim3(c.block), 0, stream, dp, c.ndesc, c.xpose1);
+        case Cmd::XPOSE:
+          v.dp = c.by_value ? nullptr : blob_dev + c.off;
+          a = {(void *)&v.dp, &v.ndesc, (void *)&c.xpose1, nullptr, nullptr, nullptr, nullptr, nullptr};
+          funcs[i] = reinterpret_cast<const void *>(&xpose_kernel);
           break;
-        }
-        case Cmd::EW: {
-          const EwDesc *dp = c.by_value ? nullptr : reinterpret_cast<const EwDesc *>(blob_dev + c.off);
-          hipLaunchKernelGGL(ew_kernel, dim3(c.grid), dim3(c.block), 0, stream, dp, c.ndesc, c.ew1);
+        case Cmd::EW:
+          v.dp = c.by_value ? nullptr : blob_dev + c.off;
+          a = {(void *)&v.dp, &v.ndesc, (void *)&c.ew1, nullptr, nullptr, nullptr, nullptr, nullptr};
+          funcs[i] = reinterpret_cast<const void *>(&ew_kernel);
           break;
+      }
+    }
+    auto node_params = [&](size_t i) {
+      hipKernelNodeParams p{};
+      p.func = const_cast<void *>(funcs[i]);
+      p.gridDim = dim3(rec.cmds[i].grid);
+      p.blockDim = dim3(rec.cmds[i].block);
+      p.sharedMemBytes = 0;
+      p.kernelParams = args[i].data();
+      p.extra = nullptr;
+      return p;
+    };
+
+    // graph replay of a repeated sequence (not while profiling per launch, not
+    // for the per-slice engine's hundreds of launches, never without allocating)
+    if (g_graph_on && !g_prof_on && !slice && !no_alloc && n >= 2) {
+      uint64_t key = 1469598103934665603ull;
+      auto mix = [&](uint64_t x) { key = (key ^ x) * 1099511628211ull; };
+      for (size_t i = 0; i < n; ++i) {
+        mix((uint64_t)(uintptr_t)funcs[i]);
+        mix((uint64_t)rec.cmds[i].grid);
+        mix((uint64_t)rec.cmds[i].block);
+      }
+      GraphEntry *e = nullptr;
+      {
+        std::lock_guard<std::mutex> g(g_graph_m);
+        for (auto &p : g_graph_list)
+          if (p->thread == std::this_thread::get_id() && p->key == key && p->dev == dev.id && p->nodes.size() == n)
+            e = p.get();
+      }
+      bool fresh = false;
+      if (!e) {
+        auto seen = std::find(g_graph_seen.begin(), g_graph_seen.end(), std::make_pair(key, dev.id));
+        if (seen == g_graph_seen.end()) {
+          g_graph_seen.push_back({key, dev.id});
+          if (g_graph_seen.size() > 32) g_graph_seen.erase(g_graph_seen.begin());
+        } else {
+          std::unique_ptr<GraphEntry> ne(new GraphEntry());
+          ne->thread = std::this_thread::get_id();
+          ne->key = key;
+          ne->dev = dev.id;
+          bool ok = hipGraphCreate(&ne->graph, 0) == hipSuccess;
+          for (size_t i = 0; ok && i < n; ++i) {
+            hipGraphNode_t node{};
+            const hipKernelNodeParams p = node_params(i);
+            ok = hipGraphAddKernelNode(&node, ne->graph, i ? &ne->nodes.back() : nullptr, i ? 1 : 0, &p) ==
+                 hipSuccess;
+            if (ok) ne->nodes.push_back(node);
+          }
+          ok = ok && hipGraphInstantiate(&ne->exec, ne->graph, nullptr, nullptr, 0) == hipSuccess;
+          if (ok) {
+            std::lock_guard<std::mutex> g(g_graph_m);
+            // keep at most kGraphsPerThread entries per thread: drop the least recently used
+            std::vector<GraphEntry *> mine;
+            for (auto &p : g_graph_list)
+              if (p->thread == ne->thread) mine.push_back(p.get());
+            if ((int)mine.size() >= kGraphsPerThread) {
+              GraphEntry *old = *std::min_element(mine.begin(), mine.end(), [](GraphEntry *x, GraphEntry *y) {
+                return x->last_use < y->last_use;
+              });
+              destroy_graph(*old);
+              g_graph_list.erase(std::find_if(g_graph_list.begin(), g_graph_list.end(),
+                                              [&](const std::unique_ptr<GraphEntry> &p) { return p.get() == old; }));
+            }
+            e = ne.get();
+            g_graph_list.push_back(std::move(ne));
+            fresh = true;
+          } else {
+            destroy_graph(*ne);
+            (void)hipGetLastError();
+          }
         }
       }
-      FG_HIP(hipGetLastError());
+      if (e) {
+        e->last_use = ++g_graph_clock;
+        bool ok = true;
+        for (size_t i = 0; ok && !fresh && i < n; ++i) {
+          const hipKernelNodeParams p = node_params(i);
+          ok = hipGraphExecKernelNodeSetParams(e->exec, e->nodes[i], &p) == hipSuccess;
+        }
+        if (ok) {
+          FG_HIP(hipGraphLaunch(e->exec, stream));
+          return FG_OK;
+        }
+        (void)hipGetLastError();  // an update failed: launch directly this time
+      }
+    }
+
+    for (size_t i = 0; i < n; ++i) {
+      Cmd &c = rec.cmds[i];
+      ProfRec pr{};
+      const bool prof = g_prof_on && c.type == Cmd::TILED;
+      if (prof) {
+        FG_HIP(hipEventCreate(&pr.start));
+        FG_HIP(hipEventCreate(&pr.stop));
+        FG_HIP(hipEventRecord(pr.start, stream));
+      }
+      FG_HIP(hipLaunchKernel(funcs[i], dim3(c.grid), dim3(c.block), args[i].data(), 0, stream));
+      if (prof) {
+        FG_HIP(hipEventRecord(pr.stop, stream));
+        pr.bytes = c.alg_bytes;
+        g_prof.push_back(pr);
+      }
     }
     return FG_OK;
   }
@@ -2387,6 +2545,11 @@ int fg_inject_status(int device, int code) {
   return FG_OK;
 }
 
+int fg_set_graphs(int on) {
+  g_graph_on = on != 0;
+  return FG_OK;
+}
+
 int fg_set_wait_limit(int spins) {
   if (spins < 0) return fail(FG_EINVAL, "wait limit must be >= 0");
   g_wait_limit = spins;
@@ -2396,6 +2559,11 @@ int fg_set_wait_limit(int spins) {
 void fg_release(void) {
   int rc = fg_profile_collect(nullptr, nullptr, nullptr);
   (void)rc;
+  {
+    std::lock_guard<std::mutex> g(g_graph_m);
+    for (auto &p : g_graph_list) destroy_graph(*p);
+    g_graph_list.clear();
+  }
   std::lock_guard<std::mutex> g(g_reg_m);
   for (auto &kv : g_reg) {
     DeviceState &s = *kv.second;

@@ -190,6 +190,14 @@ typedef struct fg_tuning {
 int fg_tuning_set(const fg_tuning *t);
 int fg_tuning_get(fg_tuning *t);
 
+/* Graph replay (no counterpart in the reference, whose host loop launches
+ * every pass, fastgeodis_cuda.cu:290-313, 604-646).  While on (the default,
+ * per host thread), a call whose sequence of launches repeats one seen before
+ * on this thread and device runs as a single hipGraphLaunch of a cached
+ * executable graph (kernel nodes only) updated with this call's arguments;
+ * results are identical either way.  Off: every launch is issued directly. */
+int fg_set_graphs(int on);
+
 /* Per-launch timing of the persistent sweep kernels, for measurement tools
  * (no counterpart in the reference).  While enabled on a host thread, every
  * tiled sweep launch it issues is bracketed by HIP events on its stream.
