@@ -156,12 +156,19 @@ class Instance:
     follow the Python wrappers (l_grad = lamb, l_eucl = 1 - lamb in double);
     for "gsf" lamb is passed raw and 1 - lamb is formed in float, as the
     reference's C++ GSF does.
+
+    image_t: optional transposed copy of ``image`` (see :func:`transposed_image`)
+    that the call uses instead of transposing the image itself (a caller that
+    transforms one image many times keeps it).
     """
 
-    __slots__ = ("image", "softmask", "spacing", "v", "lamb", "iter", "op", "theta")
+    __slots__ = ("image", "softmask", "spacing", "v", "lamb", "iter", "op", "theta", "image_t")
 
-    def __init__(self, image, softmask, v, lamb, iter=None, spacing=None, op="unsigned", theta=0.0):
+    def __init__(self, image, softmask, v, lamb, iter=None, spacing=None, op="unsigned", theta=0.0,
+                 image_t=None):
+        """See the class docstring."""
         self.image, self.softmask, self.v, self.lamb = image, softmask, v, lamb
+        self.image_t = image_t
         nd = image.dim() - 2
         self.iter = iter if iter is not None else (2 if nd == 2 else 4)
         self.spacing = list(spacing) if spacing is not None else [1.0, 1.0, 1.0]
@@ -177,6 +184,8 @@ def geodesic_batch(instances: Sequence[Instance]) -> List[torch.Tensor]:
     single-instance call with the same arguments, bit for bit.
     """
     ops = {"unsigned": OP_UNSIGNED, "signed": OP_SIGNED, "gsf": OP_GSF}
+    none = torch.empty(0)
+    images_t = [i.image_t if i.image_t is not None else none for i in instances]
     return FastGeodisCpp.geodesic_batch(
         [i.image for i in instances],
         [i.softmask for i in instances],
@@ -187,7 +196,18 @@ def geodesic_batch(instances: Sequence[Instance]) -> List[torch.Tensor]:
         [(1 - float(i.lamb)) if i.op != "gsf" else 0.0 for i in instances],
         [float(i.theta) for i in instances],
         [int(i.iter) for i in instances],
+        images_t if any(i.image_t is not None for i in instances) else [],
     )
+
+
+def transposed_image(image: torch.Tensor) -> torch.Tensor:
+    """The image in the layout of the W (3D) or column (2D) pass:
+    (1, C, W, H, D) for a (1, C, D, H, W) image, (1, C, W, H) for (1, C, H, W)."""
+    if image.dim() == 5:
+        return image.permute(0, 1, 4, 3, 2).contiguous()
+    if image.dim() == 4:
+        return image.permute(0, 1, 3, 2).contiguous()
+    raise ValueError("image must be (1, C, H, W) or (1, C, D, H, W)")
 
 
 def set_tuning(engine: int = 1, halo: int = 0, max_groups: int = 0, shape: int = 0) -> None:
@@ -234,5 +254,5 @@ def native_library_paths() -> List[str]:
 __all__ = [
     "generalised_geodesic2d", "generalised_geodesic3d",
     "signed_generalised_geodesic2d", "signed_generalised_geodesic3d",
-    "GSF2d", "GSF3d", "Instance", "geodesic_batch", "set_tuning", "get_tuning", "check_status", "set_graphs",
+    "GSF2d", "GSF3d", "Instance", "geodesic_batch", "set_tuning", "get_tuning", "check_status", "set_graphs", "transposed_image",
 ]

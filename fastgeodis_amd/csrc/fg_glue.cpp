@@ -235,10 +235,11 @@ std::vector<torch::Tensor> geodesic_batch(const std::vector<torch::Tensor> &imag
                                           const std::vector<int64_t> &ops, const std::vector<double> &vs,
                                           const std::vector<double> &lgs, const std::vector<double> &les,
                                           const std::vector<double> &thetas,
-                                          const std::vector<int64_t> &iterations) {
+                                          const std::vector<int64_t> &iterations,
+                                          const std::vector<torch::Tensor> &images_t) {
   const size_t n = images.size();
   if (masks.size() != n || spacings.size() != n || ops.size() != n || vs.size() != n || lgs.size() != n ||
-      les.size() != n || thetas.size() != n || iterations.size() != n)
+      les.size() != n || thetas.size() != n || iterations.size() != n || (!images_t.empty() && images_t.size() != n))
     throw std::invalid_argument("batch argument lists must have equal lengths");
   std::vector<torch::Tensor> outs;
   if (n == 0) return outs;
@@ -282,6 +283,17 @@ std::vector<torch::Tensor> geodesic_batch(const std::vector<torch::Tensor> &imag
     f.l_grad = (float)lgs[i];
     f.l_eucl = (float)les[i];
     f.theta = (float)thetas[i];
+    // an optional transposed image: (1, C, W, H, D) / (1, C, W, H), contiguous float32 on the same device
+    if (!images_t.empty() && images_t[i].defined() && images_t[i].numel() > 0) {
+      const torch::Tensor &t = images_t[i];
+      check_float(t, "transposed image");
+      std::vector<int64_t> want{1, ims[i].size(1)};
+      for (int k = nd + 1; k >= 2; --k) want.push_back(ims[i].size(k));
+      if (t.device() != dev || !t.is_contiguous() || t.sizes().vec() != want)
+        throw std::invalid_argument("instance " + std::to_string(i) +
+                                    ": transposed image must be a contiguous (1, C, W, H[, D]) tensor on the image's device");
+      f.imageT = t.data_ptr<float>();
+    }
     work.push_back(f);
   }
   if (work.empty()) return outs;
@@ -292,6 +304,30 @@ std::vector<torch::Tensor> geodesic_batch(const std::vector<torch::Tensor> &imag
   raise_for(fg_geodesic_batch_f32(work.data(), (int)work.size(), ws.data_ptr(), wsb, stream));
   if (sync_check_enabled()) raise_for(fg_status(stream));
   return outs;
+}
+
+// Writes values[i] at flat element indices[i] of masks[i] (float32 HIP tensors
+// of one device) on that device's current stream.
+void set_voxels(const std::vector<torch::Tensor> &masks, const std::vector<int64_t> &indices,
+                const std::vector<double> &values) {
+  const size_t n = masks.size();
+  if (indices.size() != n || values.size() != n) throw std::invalid_argument("argument lists must have equal lengths");
+  if (n == 0) return;
+  const auto dev = masks[0].device();
+  std::vector<fg_voxel_set> items(n);
+  for (size_t i = 0; i < n; ++i) {
+    const torch::Tensor &m = masks[i];
+    check_cuda(m);
+    check_float(m, "mask");
+    if (m.device() != dev || !m.is_contiguous()) throw std::invalid_argument("masks must be contiguous and on one device");
+    if (indices[i] < 0 || indices[i] >= m.numel()) throw std::invalid_argument("voxel index out of range");
+    items[i].dst = m.data_ptr<float>();
+    items[i].index = indices[i];
+    items[i].value = (float)values[i];
+  }
+  c10::DeviceGuard guard(dev);
+  void *stream = (void *)c10::hip::getCurrentHIPStream(dev.index()).stream();
+  raise_for(fg_set_voxels_f32(items.data(), (int)n, stream));
 }
 
 void set_tuning(int engine, int halo, int max_groups, int shape) {
@@ -343,7 +379,11 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("signed_generalised_geodesic3d", &signed_generalised_geodesic3d, "Signed Generalised Geodesic distance 3d");
   m.def("GSF2d", &GSF2d, "Geodesic Symmetric Filtering 2d");
   m.def("GSF3d", &GSF3d, "Geodesic Symmetric Filtering 3d");
-  m.def("geodesic_batch", &geodesic_batch, "Batched raster-scan transforms of uneven sizes");
+  m.def("geodesic_batch", &geodesic_batch, "Batched raster-scan transforms of uneven sizes",
+        pybind11::arg("images"), pybind11::arg("masks"), pybind11::arg("spacings"), pybind11::arg("ops"),
+        pybind11::arg("vs"), pybind11::arg("lgs"), pybind11::arg("les"), pybind11::arg("thetas"),
+        pybind11::arg("iterations"), pybind11::arg("images_t") = std::vector<torch::Tensor>());
+  m.def("set_voxels", &set_voxels, "Write single mask voxels (one launch per 96)");
   m.def("set_tuning", &set_tuning, "Set engine / halo depth / workgroup cap / 3D shape for this host thread");
   m.def("get_tuning", &get_tuning, "Current (engine, halo, max_groups, shape) of this host thread");
   m.def("workspace_bytes", &workspace_bytes, "Workspace bytes of one call");

@@ -888,6 +888,18 @@ __device__ __forceinline__ void ew_one(const EwDesc &E, long long i) {
   }
 }
 
+// Single-voxel writes (click edits of resident masks), passed by value in
+// groups of up to kVoxelsPerLaunch.
+constexpr int kVoxelsPerLaunch = 96;
+struct VoxelBatch {
+  fg_voxel_set it[kVoxelsPerLaunch];
+};
+
+__global__ __launch_bounds__(128) void set_voxels_kernel(VoxelBatch b, int n) {
+  const int i = threadIdx.x;
+  if (i < n) b.it[i].dst[b.it[i].index] = b.it[i].value;
+}
+
 // ---------------------------------------------------------------------------
 // Host side
 // ---------------------------------------------------------------------------
@@ -1971,6 +1983,7 @@ struct Runner {
 
 struct OpSpec {
   const float *image;
+  const float *imageT;  // the image already in the W pass layout (C, W, H, D) / (C, W, H), or null
   const float *mask;
   float *out;
   int C, Cm, D, H, W;
@@ -2071,8 +2084,8 @@ void layout_common(const std::vector<OpSpec> &ops, int K, Layout &L, Carver &cv)
 size_t op_buffers_bytes(const OpSpec &o) {
   const long long N = vox(o);
   const int nd = n_dist(o.op);
-  // image transpose (C*N) + extra distances (nd-1) + transposed distances (nd)
-  const size_t floats = (size_t)o.C * N + (size_t)(nd - 1) * N + (size_t)nd * N;
+  // image transpose (C*N, unless given) + extra distances (nd-1) + transposed distances (nd)
+  const size_t floats = (o.imageT ? 0 : (size_t)o.C * N) + (size_t)(nd - 1) * N + (size_t)nd * N;
   return floats * sizeof(float) + 16 * 256;
 }
 
@@ -2134,7 +2147,7 @@ int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream, 
     const long long N = vox(o);
     const int nd = n_dist(o.op);
     const size_t start = cv.used;
-    bufs[i].imgT = (float *)cv.take((size_t)o.C * N * sizeof(float));
+    bufs[i].imgT = o.imageT ? const_cast<float *>(o.imageT) : (float *)cv.take((size_t)o.C * N * sizeof(float));
     bufs[i].dist[0] = o.out;
     for (int k = 1; k < nd; ++k) bufs[i].dist[k] = (float *)cv.take((size_t)N * sizeof(float));
     for (int k = 0; k < nd; ++k) bufs[i].dT[k] = (float *)cv.take((size_t)N * sizeof(float));
@@ -2151,6 +2164,7 @@ int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream, 
     std::vector<XposeDesc> xs;
     for (size_t i = 0; i < ops.size(); ++i) {
       const OpSpec &o = ops[i];
+      if (o.imageT) continue;  // the caller keeps the transposed image
       if (o.is2d) xs.push_back(Runner::xd(o.image, bufs[i].imgT, o.C, o.H, 1, o.W));
       else xs.push_back(Runner::xd(o.image, bufs[i].imgT, o.C, o.D, o.H, o.W));
     }
@@ -2348,7 +2362,7 @@ int noalloc_scratch(DeviceState &ds, unsigned char **out) {
 
 extern "C" {
 
-int fg_abi_version(void) { return 2; }
+int fg_abi_version(void) { return 3; }
 
 const char *fg_last_error(void) { return g_err.c_str(); }
 
@@ -2450,6 +2464,7 @@ static int to_specs(const fg_instance *inst, int n, std::vector<OpSpec> &ops) {
     else
       ops.push_back(spec3d(f.image, f.mask, f.out, f.C, f.Cm, f.D, f.H, f.W, f.spacing, f.op, f.iterations,
                            f.v, f.l_grad, f.l_eucl, f.theta));
+    ops.back().imageT = f.imageT;
   }
   return FG_OK;
 }
@@ -2467,6 +2482,20 @@ int fg_geodesic_batch_f32(const fg_instance *inst, int n, void *workspace, size_
   if (rc) return rc;
   if (ops.empty()) return FG_OK;
   return run_ops(ops, workspace, workspace_bytes, (hipStream_t)stream);
+}
+
+int fg_set_voxels_f32(const fg_voxel_set *items, int n, void *stream) {
+  if (n < 0 || (n > 0 && !items)) return fail(FG_EINVAL, "bad voxel array");
+  for (int i = 0; i < n; ++i)
+    if (!items[i].dst || items[i].index < 0) return fail(FG_EINVAL, "voxel " + std::to_string(i) + ": bad target");
+  for (int i0 = 0; i0 < n; i0 += kVoxelsPerLaunch) {
+    VoxelBatch b;
+    const int k = std::min(kVoxelsPerLaunch, n - i0);
+    std::memcpy(b.it, items + i0, (size_t)k * sizeof(fg_voxel_set));
+    hipLaunchKernelGGL(set_voxels_kernel, dim3(1), dim3(128), 0, (hipStream_t)stream, b, k);
+    FG_HIP(hipGetLastError());
+  }
+  return FG_OK;
 }
 
 int fg_tuning_set(const fg_tuning *t) {

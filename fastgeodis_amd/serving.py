@@ -18,6 +18,12 @@ the host.  Every request's result equals the same request served alone, bit
 for bit (the batched engine computes each instance exactly as the
 single-instance call does).
 
+Everything a batch does runs on the server's stream, in order: the click
+edits of a batch (one launch for all of its voxels) are written into the
+resident masks after the previous batch's transforms have read them, so the
+masks need no per-request copies.  Each session also keeps its image in the
+transposed layout the transform's W pass reads, so no click re-transposes it.
+
 Latency is measured from a request's arrival to the completion of the HIP
 event recorded after its batch.
 """
@@ -34,7 +40,7 @@ from typing import Dict, List, Optional, Sequence, Tuple
 import numpy as np
 import torch
 
-from . import FastGeodisCpp, Instance, geodesic_batch
+from . import FastGeodisCpp, Instance, geodesic_batch, transposed_image
 
 OP_SIGNED = 1
 
@@ -44,6 +50,7 @@ class Session:
     image: torch.Tensor            # (1, C, D, H, W) float32, on device
     mask: torch.Tensor             # (1, 1, D, H, W) float32, on device
     spacing: Tuple[float, float, float]
+    image_t: Optional[torch.Tensor] = None  # (1, C, W, H, D): the image in the W pass layout
 
     @property
     def voxels(self) -> int:
@@ -75,8 +82,8 @@ class Result:
 class Server:
     """Continuous-batching server for click requests on resident sessions.
 
-    budget_bytes: device memory the in-flight requests may use (outputs,
-    mask snapshots and the batched call's workspace, estimated per request).
+    budget_bytes: device memory the in-flight requests may use (outputs and
+    the batched call's workspace, estimated per request).
     """
 
     def __init__(self, budget_bytes: int = 8 << 30, max_batch: int = 64, v: float = 1e10,
@@ -114,16 +121,17 @@ class Server:
         self._stream.wait_event(ready)
         image.record_stream(self._stream)
         mask.record_stream(self._stream)
-        self.sessions[sid] = Session(image, mask, tuple(float(s) for s in spacing))
+        with torch.cuda.stream(self._stream):
+            image_t = transposed_image(image)
+        self.sessions[sid] = Session(image, mask, tuple(float(s) for s in spacing), image_t)
 
     def request_bytes(self, sid: int) -> int:
         """Device bytes one request of this session holds while in flight:
-        output + mask snapshot + its share of the batched workspace
-        (transposed image, second distance, two transposed distances)."""
+        output + its share of the batched workspace (second distance, two
+        transposed distances; the session keeps its transposed image)."""
         s = self.sessions[sid]
         n = s.voxels
-        C = int(s.image.shape[1])
-        return 4 * n * (2 + C + 3) + (1 << 20)
+        return 4 * n * (1 + 3) + (1 << 20)
 
     # ---- requests -------------------------------------------------------
     def submit(self, click: Click) -> Future:
@@ -177,24 +185,30 @@ class Server:
         return batch
 
     def _launch(self, batch):
-        insts, snaps = [], []
+        insts, masks, idx, vals = [], [], [], []
+        for click, _ in batch:
+            s = self.sessions[click.session]
+            _, _, D, H, W = s.mask.shape
+            val = 1.0 if click.foreground else 0.0
+            for (z, y, x) in click.voxels:
+                if not (0 <= z < D and 0 <= y < H and 0 <= x < W):
+                    raise ValueError(f"click voxel {(z, y, x)} outside session {click.session}")
+                masks.append(s.mask)
+                idx.append((z * H + y) * W + x)
+                vals.append(val)
+            # at most one click per session in a batch: the mask as edited is
+            # this request's view of it
+            insts.append(Instance(s.image, s.mask, self.v, self.lamb, iter=self.iterations,
+                                  spacing=list(s.spacing), op="signed", image_t=s.image_t))
         with torch.cuda.stream(self._stream):
-            for click, _ in batch:
-                s = self.sessions[click.session]
-                val = 1.0 if click.foreground else 0.0
-                for (z, y, x) in click.voxels:
-                    s.mask[0, 0, z, y, x] = val
-                snap = s.mask.clone()  # this request's view of the mask
-                snaps.append(snap)
-                insts.append(Instance(s.image, snap, self.v, self.lamb, iter=self.iterations,
-                                      spacing=list(s.spacing), op="signed"))
+            FastGeodisCpp.set_voxels(masks, idx, vals)
             outs = geodesic_batch(insts)
             ev = torch.cuda.Event()
             ev.record(self._stream)
-        return outs, ev, snaps
+        return outs, ev
 
     def _run(self):
-        pending = collections.deque()  # (batch, outs, event, snaps)
+        pending = collections.deque()  # (batch, outs, event)
         try:
             self._loop(pending)
         except BaseException as exc:  # fail every outstanding request instead of hanging it
@@ -219,16 +233,16 @@ class Server:
                 batch = self._admit() if len(pending) < self.inflight_limit else []
             if batch:
                 try:
-                    outs, ev, snaps = self._launch(batch)
+                    outs, ev = self._launch(batch)
                 except BaseException as exc:
                     for _, f in batch:
                         f.set_exception(exc)
                     raise
-                pending.append((batch, outs, ev, snaps))
+                pending.append((batch, outs, ev))
                 self.batches.append(len(batch))
                 continue
             if pending:
-                b, outs, ev, snaps = pending[0]
+                b, outs, ev = pending[0]
                 if not ev.query():
                     # let new arrivals be admitted while the device is busy
                     time.sleep(20e-6)

@@ -166,11 +166,26 @@ typedef struct fg_instance {
   float l_grad;       /* unsigned/signed: lambda; GSF: lambda */
   float l_eucl;       /* unsigned/signed: 1 - lambda; GSF: ignored */
   float theta;        /* GSF only */
+  const float *imageT; /* optional (else NULL): the image already transposed
+                          to (C, W, H, D) (3D) or (C, W, H) (2D), e.g. kept by
+                          a caller that transforms one image many times; the
+                          call then skips its image transpose and workspace */
 } fg_instance;
 
 size_t fg_batch_workspace_bytes(const fg_instance *inst, int n);
 int fg_geodesic_batch_f32(const fg_instance *inst, int n, void *workspace,
                           size_t workspace_bytes, void *stream);
+
+/* Writes n single float values (items[i].dst[items[i].index] = value),
+ * asynchronously on `stream`, in one launch per 96 items; the items are
+ * passed by value (no device allocation).  No counterpart in the reference:
+ * it applies click edits to resident masks in the serving loop. */
+typedef struct fg_voxel_set {
+  float *dst;
+  long long index;
+  float value;
+} fg_voxel_set;
+int fg_set_voxels_f32(const fg_voxel_set *items, int n, void *stream);
 
 /* Tile/launch parameters of the persistent sweep engine.  engine 0 is the
  * one-launch-per-slice engine, 1 the persistent tiled engine (default).

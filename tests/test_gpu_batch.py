@@ -148,3 +148,52 @@ def test_shared_halo_planning_mixed_rings(device, seed):
     torch.cuda.synchronize()
     for k, (inst, out) in enumerate(zip(insts, outs)):
         assert bits_equal(out, single(inst)), f"instance {k} ({inst.op}, {tuple(inst.image.shape)}) differs"
+
+
+def test_transposed_image_given_equals_computed(device):
+    """Instances that bring their image already transposed (as the serving
+    loop's sessions do) give the same bits as instances that do not."""
+    rng = np.random.default_rng(21)
+    insts, with_t = [], []
+    for k in range(4):
+        if k % 2:
+            shape = (int(rng.integers(10, 60)), int(rng.integers(10, 90)))
+            inst = rand_instance(rng, 2, 1 + k % 2, shape, ["unsigned", "signed"][k % 2], 0.5, 2)
+        else:
+            shape = tuple(int(x) for x in rng.integers(6, 40, size=3))
+            inst = rand_instance(rng, 3, 1 + k % 2, shape, ["signed", "gsf"][k // 2 % 2], 1.0, 2,
+                                 [1.5, 1.0, 0.7])
+        insts.append(inst)
+        with_t.append(FG.Instance(inst.image, inst.softmask, inst.v, inst.lamb, iter=inst.iter,
+                                  spacing=inst.spacing, op=inst.op, theta=inst.theta,
+                                  image_t=FG.transposed_image(inst.image)))
+    a = FG.geodesic_batch(insts)
+    b = FG.geodesic_batch(with_t)
+    for k, (x, y) in enumerate(zip(a, b)):
+        assert bits_equal(x, y), f"instance {k}"
+    bad = FG.Instance(insts[0].image, insts[0].softmask, 1e10, 1.0, spacing=insts[0].spacing,
+                      image_t=insts[0].image.clone())  # wrong layout
+    if insts[0].image.shape[2] != insts[0].image.shape[4]:
+        with pytest.raises(ValueError):
+            FG.geodesic_batch([bad])
+
+
+def test_set_voxels_writes_single_values(device):
+    a = torch.zeros(3, 40, 50, device="cuda")
+    b = torch.full((1, 1, 7, 9, 11), 0.5, device="cuda")
+    idx = [0, 5, 119, 3 * 40 * 50 - 1]
+    FG.FastGeodisCpp.set_voxels([a] * 4 + [b] * 2, idx + [0, 7 * 9 * 11 - 1], [1.0, 2.0, 3.0, 4.0, 0.0, 1.0])
+    want = torch.zeros(3, 40, 50)
+    want.view(-1)[idx] = torch.tensor([1.0, 2.0, 3.0, 4.0])
+    assert torch.equal(a.cpu(), want)
+    wb = torch.full((1, 1, 7, 9, 11), 0.5)
+    wb.view(-1)[0], wb.view(-1)[-1] = 0.0, 1.0
+    assert torch.equal(b.cpu(), wb)
+    with pytest.raises(ValueError):
+        FG.FastGeodisCpp.set_voxels([a], [3 * 40 * 50], [1.0])
+    # more than one launch's worth of items
+    c = torch.zeros(1000, device="cuda")
+    FG.FastGeodisCpp.set_voxels([c] * 300, list(range(0, 600, 2)), [float(i) for i in range(300)])
+    wc = torch.zeros(1000)
+    wc[0:600:2] = torch.arange(300, dtype=torch.float32)
+    assert torch.equal(c.cpu(), wc)
