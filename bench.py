@@ -85,6 +85,30 @@ def _median_time(fn, runs: int) -> float:
     return float(np.median(times))
 
 
+def exchange_summary(hist, khz) -> dict:
+    """Percentiles of the per-wave halo-exchange time from the log2 histogram
+    of device-clock ticks (a bucket's upper edge is reported)."""
+    total = sum(hist)
+    if total == 0 or khz <= 0:
+        return {"exchanges": 0}
+    us_per_tick = 1e3 / khz
+
+    def pct(q):
+        acc = 0
+        for b, c in enumerate(hist):
+            acc += c
+            if acc >= q * total:
+                return (2 ** b) * us_per_tick
+        return (2 ** (len(hist) - 1)) * us_per_tick
+
+    mean = sum(c * (0 if b == 0 else 1.5 * 2 ** (b - 1)) for b, c in enumerate(hist)) / total * us_per_tick
+    return {"wave_exchanges": total, "p50_us_le": round(pct(0.5), 3), "p90_us_le": round(pct(0.9), 3),
+            "p99_us_le": round(pct(0.99), 3), "mean_us_approx": round(mean, 3),
+            "histogram_log2_ticks": hist, "clock_khz": khz,
+            "what": "per wave and exchange: time from publishing its halo band to receiving the last "
+                    "neighbour value (device constant-rate clock, log2 buckets; p = bucket upper edge)"}
+
+
 def host_fields() -> dict:
     try:
         affinity = len(os.sched_getaffinity(0))
@@ -230,12 +254,18 @@ def main():
     # per-launch durations of the sweep kernel: HIP events around each launch
     # on its own stream, in separate (untimed) calls
     prof_steps = max(1, min(args.steps, 3))
-    FG.FastGeodisCpp.profile_enable(True)
+    FG.FastGeodisCpp.profile_enable(1)
     for k in range(prof_steps):
         step()
-    FG.FastGeodisCpp.profile_enable(False)
+    FG.FastGeodisCpp.profile_enable(0)
     launches, kern_ms, kern_bytes = FG.FastGeodisCpp.profile_collect()
+    # halo-exchange waits of the sweeps (device timestamps), in their own call
+    FG.FastGeodisCpp.profile_enable(2)
+    step()
+    FG.FastGeodisCpp.profile_enable(0)
+    hist, khz = FG.FastGeodisCpp.profile_exchange_histogram()
     FG.check_status()
+    exchange = exchange_summary(hist, khz)
 
     avg_ms = kern_ms / max(launches, 1)
     bytes_per_launch = kern_bytes / max(launches, 1)
@@ -328,6 +358,7 @@ def main():
                    "halo": tuning["halo"], "tile_shape": tuning["shape"]},
         "whole_call_alg_GBps": round(whole_gbs, 1),
         "roofline": roofline,
+        "exchange_wait": exchange,
         "hbm_copy": copy_res,
         "cpu_baseline": cpu,
         "small_2d_call": c1,

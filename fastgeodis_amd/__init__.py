@@ -177,12 +177,25 @@ class Instance:
         self.op, self.theta = op, theta
 
 
-def geodesic_batch(instances: Sequence[Instance]) -> List[torch.Tensor]:
+def geodesic_batch(instances: Sequence[Instance], tuned: bool = False) -> List[torch.Tensor]:
     """Runs many 2D/3D instances of uneven sizes in the same persistent launches.
 
     All tensors must be on one HIP device.  Each returned tensor equals the
-    single-instance call with the same arguments, bit for bit.
+    single-instance call with the same arguments, bit for bit.  ``tuned``:
+    use the tuning that :mod:`fastgeodis_amd.autotune` measured for the
+    batch's size class (measuring it on first use) instead of the analytic
+    defaults; results are identical either way.
     """
+    if tuned and instances:
+        from . import autotune
+
+        best = autotune.tuning_for_batch(instances, device=instances[0].image.device)
+        old = get_tuning()
+        set_tuning(best["engine"], best["halo"], best["max_groups"], best["shape"])
+        try:
+            return geodesic_batch(instances)
+        finally:
+            set_tuning(**old)
     ops = {"unsigned": OP_UNSIGNED, "signed": OP_SIGNED, "gsf": OP_GSF}
     none = torch.empty(0)
     images_t = [i.image_t if i.image_t is not None else none for i in instances]
@@ -215,7 +228,9 @@ def set_tuning(engine: int = 1, halo: int = 0, max_groups: int = 0, shape: int =
     neighbour exchanges; 0 = automatic: the deepest halo whose tiles fit one
     per compute unit), workgroup cap (0 = occupancy limit) and 3D plane
     workgroup shape (0 auto = 5, 1 = 4 waves x 8 rows, 2 = 8 x 4, 3 = 16 x 2,
-    4 = 8 x 4 with a deeper prefetch ring, 5 = 16 x 2 with the deeper ring)
+    4 = 8 x 4 with a deeper prefetch ring, 5 = 16 x 2 with the deeper ring,
+    6 = 8 x 4 with the deeper ring and one more wave that performs the halo
+    exchanges)
     used by later calls on this host thread."""
     FastGeodisCpp.set_tuning(int(engine), int(halo), int(max_groups), int(shape))
 

@@ -14,6 +14,14 @@ file.
     from fastgeodis_amd import autotune
     best = autotune.tune((128, 256, 256), channels=1, spacing=(2.5, 1, 1))
     autotune.apply(best)            # tuning for later calls on this thread
+
+Batched calls are tuned as a whole: :func:`tune_batch` times the complete
+batched call under each candidate tuning (halo depth, 3D workgroup shape,
+cap on co-resident workgroups per launch, which splits a large batch into
+more launches) and caches the fastest per *size class* of the batch
+(:func:`batch_signature`: the multiset of instance kinds with extents rounded
+to powers of two, and the device).  ``geodesic_batch(..., tuned=True)``
+applies the cached tuning of its size class, measuring it first when absent.
 """
 from __future__ import annotations
 
@@ -155,3 +163,99 @@ def apply(result: dict) -> None:
     """Makes a :func:`tune` result the tuning of later calls on this thread."""
     b = result["best"]
     set_tuning(int(b["engine"]), int(b["halo"]), 0, int(b["shape"]))
+
+
+# ---- whole batches ----------------------------------------------------------
+
+# (engine, halo, shape, max_groups): halo 0 = automatic, shape 0 = default,
+# max_groups 0 = every co-resident workgroup, < 0 = that fraction of the
+# compute units (-2: half)
+DEFAULT_BATCH_CANDIDATES: Tuple[Tuple[int, int, int, int], ...] = (
+    (1, 0, 0, 0), (1, 0, 4, 0), (1, 0, 2, 0), (1, 5, 0, 0), (1, 10, 0, 0), (1, 0, 0, -2),
+)
+
+
+def _bucket(x: int) -> int:
+    """Power of two nearest to x (in log scale)."""
+    import math
+
+    return 1 << max(0, int(round(math.log2(max(int(x), 1)))))
+
+
+def batch_signature(instances, device="cuda") -> str:
+    """Size class of a batch: sorted (dims, C, op, variant class, iterations,
+    extents rounded to powers of two) of every instance, and the device."""
+    items = []
+    for i in instances:
+        ext = tuple(_bucket(x) for x in i.image.shape[2:])
+        items.append(f"{len(ext)}d{'x'.join(map(str, ext))}c{int(i.image.shape[1])}{i.op[0]}"
+                     f"{variant_class(i.lamb)}i{int(i.iter)}")
+    items.sort()
+    import hashlib
+
+    h = hashlib.sha1(",".join(items).encode()).hexdigest()[:16]
+    return f"batch:{len(items)}:{h}:{device_tag(device)}"
+
+
+def _resolve_groups(g: int, device) -> int:
+    if g >= 0:
+        return g
+    cus = torch.cuda.get_device_properties(torch.device(device)).multi_processor_count
+    return max(1, cus // (-g))
+
+
+def tune_batch(instances, reps: int = 2, candidates=None, device="cuda", use_cache: bool = True) -> dict:
+    """Times the batched call of `instances` under each candidate tuning and
+    returns ``{"signature", "best": {"engine", "halo", "shape", "max_groups"},
+    "ms", "analytic_ms", "table"}``; every candidate must return the same
+    bits as the analytic default (the first candidate)."""
+    from . import geodesic_batch
+
+    sig = batch_signature(instances, device)
+    if use_cache and sig in load_cache():
+        return _cache[sig]
+    cands = list(candidates if candidates is not None else DEFAULT_BATCH_CANDIDATES)
+    old = get_tuning()
+    table: List[dict] = []
+    ref = None
+    try:
+        for (engine, halo, shp, groups) in cands:
+            g = _resolve_groups(groups, device)
+            set_tuning(engine, halo, g, shp)
+            outs = geodesic_batch(instances)
+            torch.cuda.synchronize()
+            if ref is None:
+                ref = [o.clone() for o in outs]
+            elif not all(torch.equal(a.view(torch.int32), b.view(torch.int32)) for a, b in zip(ref, outs)):
+                raise RuntimeError(f"batch tuning {(engine, halo, shp, g)} changed a result")
+            del outs
+        for (engine, halo, shp, groups) in cands:
+            g = _resolve_groups(groups, device)
+            set_tuning(engine, halo, g, shp)
+            geodesic_batch(instances)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                geodesic_batch(instances)
+            torch.cuda.synchronize()
+            table.append({"engine": engine, "halo": halo, "shape": shp, "max_groups": g,
+                          "ms": (time.perf_counter() - t0) * 1e3 / reps})
+    finally:
+        set_tuning(**old)
+    best = min(table, key=lambda r: r["ms"])
+    res = {"signature": sig, "best": {k: best[k] for k in ("engine", "halo", "shape", "max_groups")},
+           "ms": best["ms"], "analytic_ms": table[0]["ms"], "table": table}
+    _cache[sig] = res
+    _save_cache()
+    return res
+
+
+def tuning_for_batch(instances, device="cuda", measure: bool = True) -> Optional[dict]:
+    """The cached tuning of the batch's size class ({"engine", "halo",
+    "shape", "max_groups"}); measured now when absent and `measure`."""
+    sig = batch_signature(instances, device)
+    if sig in load_cache():
+        return _cache[sig]["best"]
+    if not measure:
+        return None
+    return tune_batch(instances, device=device)["best"]

@@ -341,7 +341,8 @@ std::vector<int64_t> get_tuning() {
   return {t.engine, t.halo, t.max_groups, t.shape};
 }
 
-void profile_enable(bool on) { raise_for(fg_profile_enable(on ? 1 : 0)); }
+// mode: False/0 off, True/1 launch events, 2 exchange-wait histogram, 3 both
+void profile_enable(int64_t mode) { raise_for(fg_profile_enable((int)mode)); }
 
 std::tuple<int64_t, double, double> profile_collect() {
   int n = 0;
@@ -365,6 +366,16 @@ void inject_status(int64_t device, int64_t code) { raise_for(fg_inject_status((i
 void set_wait_limit(int64_t spins) { raise_for(fg_set_wait_limit((int)spins)); }
 
 void set_graphs(bool on) { raise_for(fg_set_graphs(on ? 1 : 0)); }
+
+// (counts per log2 bucket of device-clock ticks, device clock rate in kHz)
+std::tuple<std::vector<int64_t>, int64_t> profile_exchange_histogram() {
+  unsigned h[32] = {};
+  raise_for(fg_profile_exchange_histogram(h, 32));
+  int dev = 0, khz = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev);
+  return {std::vector<int64_t>(h, h + 32), (int64_t)khz};
+}
 
 int64_t workspace_bytes(int64_t op, int64_t C, int64_t D, int64_t H, int64_t W) {
   return (int64_t)fg_workspace_bytes((int)op, (int)C, (int)D, (int)H, (int)W);
@@ -390,6 +401,8 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("abi_version", []() { return fg_abi_version(); });
   m.def("profile_enable", &profile_enable, "Bracket tiled sweep launches with HIP events");
   m.def("profile_collect", &profile_collect, "(launches, total_ms, algorithmic_bytes) since enabling");
+  m.def("profile_exchange_histogram", &profile_exchange_histogram,
+        "(counts per log2 bucket of device-clock ticks, clock kHz) of halo-exchange waits since enabling");
   m.def("check_status", &check_status, "Synchronise the device's current stream and raise on a pending device error");
   m.def("pending_status", &pending_status, "Read and clear a device's pending status code without synchronising");
   m.def("inject_status", &inject_status, "Set a device's pending status from the host (error-path tests)");

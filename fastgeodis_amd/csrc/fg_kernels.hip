@@ -69,6 +69,11 @@ constexpr int kMaxK = 24;
 // s_sleep units (64 clocks each) between publishing a halo and first polling
 // the neighbours' publications.
 constexpr int kHaloPollDelay = 24;
+// Waves that perform the halo exchanges in kernels with exchange waves.
+constexpr int kCommWaves = 4;
+// Buckets of the exchange-wait histogram: bucket b counts waits of
+// [2^(b-1), 2^b) ticks of the constant-rate device clock (bucket 0: 0 ticks).
+constexpr int kXHistBuckets = 32;
 
 // ---------------------------------------------------------------------------
 // Descriptors
@@ -240,10 +245,18 @@ struct Slab {  // one prefetched slice: image rows -1..R, distance rows 0..R-1
 // consecutive B.
 // INIT: the first sweep of a transform, whose input distance is computed from
 // the mask as it is loaded (SweepDesc::init_mode).
-template <int NW, int R, int P, int CT, bool LAMB1, bool FAST, bool XST = false, bool INIT = false>
-__global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
+// COMM: kCommWaves more waves (indices NW, NW + 1, ...) that hold no rows and
+// perform every halo exchange of the tile, each for its share of the window
+// rows: the compute waves leave their front in LDS and wait at a barrier while
+// they publish and poll, so the compute waves' prefetch loads stay in flight
+// (a wave's wait for a halo load is also a wait for every load it issued
+// earlier).
+template <int NW, int R, int P, int CT, bool LAMB1, bool FAST, bool XST = false, bool INIT = false,
+          bool COMM = false>
+__global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_tiled_kernel(
     const SweepDesc *__restrict__ descs, int ndesc, SweepDesc one, int K, unsigned tag_base,
-    unsigned long long *__restrict__ halo_base, int *__restrict__ status, int spin_limit) {
+    unsigned long long *__restrict__ halo_base, int *__restrict__ status, int spin_limit,
+    unsigned *__restrict__ xhist) {
   // LDS slot = ring position of the step (a compile-time index in the
   // unrolled loops, so edge addresses are immediate offsets)
   constexpr int NWL = NW > 1 ? NW : 1;
@@ -254,10 +267,17 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   constexpr int NWR = NW * R;  // window rows
   static_assert(!XST || (NW > 1 && 64 % NWR == 0), "transposed output needs a multi-wave window of 16/32/64 rows");
   __shared__ float xtile[XST ? P : 1][XST ? NWR : 1][65];
+  __shared__ unsigned xhist_wg[kXHistBuckets];  // this workgroup's exchange-wait counts (measurement)
+  static_assert(!COMM || (NW > 1 && NWR % kCommWaves == 0), "exchange waves serve multi-wave sweeps");
+  __shared__ float xfront[COMM ? NWR : 1][64];     // COMM: the front around an exchange
 
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // wave index: uniform
   const int tile = xcd_remap(blockIdx.x, gridDim.x);
+  if (xhist) {
+    if (threadIdx.x < kXHistBuckets) xhist_wg[threadIdx.x] = 0u;
+    __syncthreads();
+  }
 
   // descriptor owning this tile: the launch's only one (passed by value when
   // descs is null) or a binary search on tile0 (uniform branches and loads,
@@ -311,6 +331,89 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   const int nsteps = nA - 1;
   if (nsteps <= 0) return;
   const int a_first = (dir > 0) ? 1 : nA - 2;
+  const int nblocks_all = (nsteps + K - 1) / K;
+
+  // The exchange wave: the same barrier sequence as the compute waves (one
+  // per step, two per exchange), publishing and polling between the two
+  // barriers of each exchange for every row of the window.
+  if constexpr (COMM) {
+    if (w >= NW) {
+      constexpr int CR = NWR / kCommWaves;  // window rows per exchange wave
+      const int cr0 = (w - NW) * CR;
+      bool tripped = false;
+      for (int blk = 0;; ++blk) {
+        for (int st = 0; st < K; ++st) __syncthreads();
+        if (blk + 1 >= nblocks_all) break;
+        __syncthreads();  // the compute waves' front is in xfront
+        const unsigned long long t_in = xhist ? wall_clock64() : 0ull;
+        const unsigned tag = tag_base + (unsigned)(blk + 1);
+        const long long hslot = (long long)((blk + 1) & 1) * tile_slots;
+        gu64 *pub = halo + (long long)lt * 2 * tile_slots + hslot;
+        const int own_w = e_own1 - e_own0;
+        const bool lane_band = (te > 0 && e - e_own0 < K) || (te < nTE - 1 && e_own1 - 1 - e < K);
+        const int lane_dist = max(max(e_own0 - e, e - (e_own1 - 1)), 0);
+#pragma unroll
+        for (int k = 0; k < CR; ++k) {
+          const int r = cr0 + k;
+          const int b = bw0 + r;
+          const bool row_band = (tb > 0 && b - b_own0 < K) || (tb < nTB - 1 && b_own1 - 1 - b < K);
+          if (b >= b_own0 && b < b_own1 && lane_own && (row_band || lane_band))
+            __hip_atomic_store(pub + (long long)(b - b_own0) * own_w + lane_pub, pack(xfront[r][lane], tag),
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __builtin_amdgcn_s_sleep(kHaloPollDelay);
+        // whether this lane of window row r needs a neighbour's value, and
+        // where that neighbour publishes it (recomputed rather than held, so
+        // only the 32 loaded words stay live)
+        auto row_need = [&](int r) {
+          const int b = bw0 + r;
+          const bool row_own = b >= b_own0 && b < b_own1;
+          const int row_dist = max(max(b_own0 - b, b - (b_own1 - 1)), 0);
+          return b >= 0 && b < nB && lane_valid && !(row_own && lane_own) && max(row_dist, lane_dist) <= K;
+        };
+        auto row_src = [&](int r) -> const gu64 * {
+          const int bq = min(max(bw0 + r, 0), nB - 1);
+          const int otb = bq / TB;  // uniform
+          return halo + (long long)(otb * nTE + ote) * 2 * tile_slots + hslot + (long long)(bq - otb * TB) * ow +
+                 lane_src;
+        };
+        unsigned long long x[CR];
+#pragma unroll
+        for (int k = 0; k < CR; ++k) {
+          x[k] = (unsigned long long)tag << 32;  // lanes that need nothing count as arrived
+          if (row_need(cr0 + k)) x[k] = __hip_atomic_load(row_src(cr0 + k), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        int spins = tripped ? spin_limit : 0;
+        while (true) {
+          bool waiting = false;
+#pragma unroll
+          for (int k = 0; k < CR; ++k) waiting = waiting || (unsigned)(x[k] >> 32) != tag;
+          if (!waiting) break;
+          if (++spins > spin_limit) {
+            __hip_atomic_store(status, FG_EWATCHDOG, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            tripped = true;
+            break;
+          }
+#pragma unroll
+          for (int k = 0; k < CR; ++k)
+            if ((unsigned)(x[k] >> 32) != tag)
+              x[k] = __hip_atomic_load(row_src(cr0 + k), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+#pragma unroll
+        for (int k = 0; k < CR; ++k)
+          if (row_need(cr0 + k)) xfront[cr0 + k][lane] = __uint_as_float((unsigned)x[k]);
+        if (xhist && lane == 0) {
+          const unsigned long long dt = wall_clock64() - t_in;
+          const int bucket = dt ? min(64 - __clzll(dt), kXHistBuckets - 1) : 0;
+          atomicAdd(&xhist_wg[bucket], 1u);
+        }
+        __syncthreads();  // the halo is in xfront
+      }
+      if constexpr (XST) __syncthreads();  // the last staged slice
+      if (xhist) __syncthreads();           // the workgroup's histogram flush
+      return;
+    }
+  }
 
   // Row bookkeeping: rows -1..R relative to this wave.  rofs[r] is the byte
   // offset of this lane's element of row r (clamped into the plane) within
@@ -449,7 +552,29 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
   // wait for the new prefetch; pending rows are polled together, so one
   // memory round trip serves all of them.
   bool wd_tripped = false;
+  // COMM: hand the front to the exchange wave and take the halo back from it
+  auto exchange_via_comm = [&](int blk, Slab<R, CT> &pf_slab, int pf_step) {
+    (void)blk;
+#pragma unroll
+    for (int r = 0; r < R; ++r) xfront[w * R + r][lane] = Fd[r];
+    load_slab(pf_slab, min(max(a_first + pf_step * dir, 0), nA - 1));
+    __syncthreads();  // front visible to the exchange wave
+    __syncthreads();  // halo values written by it
+    const int lane_dist = max(max(e_own0 - e, e - (e_own1 - 1)), 0);
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int b = rb[r + 1];
+      const bool row_own = b >= b_own0 && b < b_own1;
+      const int row_dist = max(max(b_own0 - b, b - (b_own1 - 1)), 0);
+      const bool need = rvalid[r + 1] && lane_valid && !(row_own && lane_own) && max(row_dist, lane_dist) <= K;
+      if (need) Fd[r] = xfront[w * R + r][lane];
+    }
+  };
   auto exchange = [&](int blk, Slab<R, CT> &pf_slab, int pf_step) {
+    // measurement only (xhist non-null): how long this wave spends in the
+    // exchange, from its publications to the last halo value it waits for,
+    // as a histogram of the device's constant-rate clock (log2 buckets)
+    const unsigned long long t_in = xhist ? wall_clock64() : 0ull;
     const unsigned tag = tag_base + (unsigned)blk;
     const long long hslot = (long long)(blk & 1) * tile_slots;
     gu64 *pub = halo + (long long)lt * 2 * tile_slots + hslot;
@@ -510,6 +635,11 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
 #pragma unroll
     for (int r = 0; r < R; ++r)
       if (need[r]) Fd[r] = __uint_as_float((unsigned)x[r]);
+    if (xhist && lane == 0) {
+      const unsigned long long dt = wall_clock64() - t_in;
+      const int bucket = dt ? min(64 - __clzll(dt), kXHistBuckets - 1) : 0;
+      atomicAdd(&xhist_wg[bucket], 1u);  // LDS; added to the device histogram at the end
+    }
   };
 
   // One slice: Sl holds the new slice, Fr the previous one (front image).
@@ -604,11 +734,16 @@ __global__ __launch_bounds__(NW * 64) void sweep_tiled_kernel(
     step(S[P - 1], S[P - 2], sl + P - 1, P - 1, false);
     if (blk + 1 >= nblocks) break;
     // deferred prefetch of step s0 + K - 1: the slice of step s0 + K + P - 2
-    exchange(blk + 1, S[P - 2], s0 + K + P - 2);
+    if constexpr (COMM) exchange_via_comm(blk + 1, S[P - 2], s0 + K + P - 2);
+    else exchange(blk + 1, S[P - 2], s0 + K + P - 2);
   }
   if constexpr (XST) {
     __syncthreads();
     xflush(P - 1, nblocks * K - 1);  // the last step sits in slot P - 1
+  }
+  if (xhist) {
+    __syncthreads();
+    if (threadIdx.x < kXHistBuckets && xhist_wg[threadIdx.x]) atomicAdd(xhist + threadIdx.x, xhist_wg[threadIdx.x]);
   }
 }
 
@@ -915,7 +1050,8 @@ struct ProfRec {
   hipEvent_t start, stop;
   double bytes;
 };
-thread_local bool g_prof_on = false;
+thread_local bool g_prof_on = false;     // events around tiled launches
+thread_local bool g_prof_xhist = false;  // exchange-wait histogram
 // Checks of a pending halo value before a wait expires (0 = default); set by
 // fg_set_wait_limit.
 thread_local int g_wait_limit = 0;
@@ -988,6 +1124,7 @@ int first_pass(const Xform &x) { return x.is2d ? 1 : 0; }
 // Tiled plan for one sweep plane.
 struct Plan {
   int NW, R;
+  bool comm;  // one more wave that performs the halo exchanges
   int P;  // slab ring size of the kernel that runs this plan
   int K;  // halo depth actually used: the requested depth rounded up to a multiple of P
   int nTB, nTE, TB, TE;
@@ -997,11 +1134,14 @@ struct Plan {
 struct Shape {
   int NW, R;
   int P12, P34;  // slab ring size for 1-2 / 3-4 image channels (prefetch distance P - 1)
+  bool comm;     // plus kCommWaves waves that perform the halo exchanges
 };
 // Workgroup shapes with compiled kernels: index 0 serves 2D row sweeps
 // (one row); the others tile 3D planes (selected by fg_tuning.shape).
-constexpr Shape kShapes[] = {{1, 1, 8, 8}, {4, 8, 2, 2}, {8, 4, 3, 2}, {16, 2, 3, 3}, {8, 4, 5, 2}, {16, 2, 5, 3}};
-constexpr int kNumShapes = 6;
+constexpr Shape kShapes[] = {{1, 1, 8, 8, false},  {4, 8, 2, 2, false},  {8, 4, 3, 2, false},
+                             {16, 2, 3, 3, false}, {8, 4, 5, 2, false},  {16, 2, 5, 3, false},
+                             {8, 4, 5, 2, true}};
+constexpr int kNumShapes = 7;
 constexpr int kAutoShape = 5;  // fastest on 512^2 planes: 16 x 2 with the deepest ring that keeps K = 5
 thread_local int g_shape = kAutoShape;
 
@@ -1011,6 +1151,7 @@ Plan plan_fixed(int nB, int nE, int Kreq, int C) {
   const Shape s = (nB == 1) ? kShapes[0] : kShapes[g_shape];
   p.NW = s.NW;
   p.R = s.R;
+  p.comm = s.comm && C <= 2;  // exchange-wave kernels exist for 1-2 channels; 3-4 run the plain 8 x 4
   const int P = C <= 2 ? s.P12 : s.P34;
   p.P = P;
   const int K = ((Kreq + P - 1) / P) * P;
@@ -1070,22 +1211,38 @@ void pass_plane(const Xform &x, int pass, int &nA, int &nB, int &nE) {
   else { nA = x.W; nB = x.H; nE = x.D; }
 }
 
-template <int NW, int R, int P, int CT, bool L1, bool FA, bool XS = false, bool IN = false>
+template <int NW, int R, int P, int CT, bool L1, bool FA, bool XS = false, bool IN = false, bool CM = false>
 const void *kernel_ptr() {
-  return reinterpret_cast<const void *>(&sweep_tiled_kernel<NW, R, P, CT, L1, FA, XS, IN>);
+  return reinterpret_cast<const void *>(&sweep_tiled_kernel<NW, R, P, CT, L1, FA, XS, IN, CM>);
 }
 
 // lamb1 implies fast (l_grad == 1 and l_eucl*LD == +0 are non-negative).
 // Only the channel counts a (shape, ring) pair serves are instantiated.
-template <int NW, int R, int P, int CT, bool XS = false, bool IN = false>
+template <int NW, int R, int P, int CT, bool XS = false, bool IN = false, bool CM = false>
 const void *pick_variant(bool lamb1, bool fast) {
-  if (lamb1) return kernel_ptr<NW, R, P, CT, true, true, XS, IN>();
-  return fast ? kernel_ptr<NW, R, P, CT, false, true, XS, IN>() : kernel_ptr<NW, R, P, CT, false, false, XS, IN>();
+  if (lamb1) return kernel_ptr<NW, R, P, CT, true, true, XS, IN, CM>();
+  return fast ? kernel_ptr<NW, R, P, CT, false, true, XS, IN, CM>()
+              : kernel_ptr<NW, R, P, CT, false, false, XS, IN, CM>();
 }
 
 template <int NW, int R, int P, int CT>
 const void *pick_variant_init(bool lamb1, bool fast, bool init) {
   return init ? pick_variant<NW, R, P, CT, false, true>(lamb1, fast) : pick_variant<NW, R, P, CT>(lamb1, fast);
+}
+
+// Kernels of the 8 x 4 shape with an exchange wave (ring 5, 1-2 channels).
+const void *pick_comm_kernel(int C, bool lamb1, bool fast, bool xst, bool init) {
+  if (C == 1) {
+    if (xst) return pick_variant<8, 4, 5, 1, true, false, true>(lamb1, fast);
+    return init ? pick_variant<8, 4, 5, 1, false, true, true>(lamb1, fast)
+                : pick_variant<8, 4, 5, 1, false, false, true>(lamb1, fast);
+  }
+  if (C == 2) {
+    if (xst) return pick_variant<8, 4, 5, 2, true, false, true>(lamb1, fast);
+    return init ? pick_variant<8, 4, 5, 2, false, true, true>(lamb1, fast)
+                : pick_variant<8, 4, 5, 2, false, false, true>(lamb1, fast);
+  }
+  return nullptr;
 }
 
 template <int NW, int R, int P, int CLO, int CHI>
@@ -1105,8 +1262,10 @@ const void *pick_kernel_nr(int C, bool lamb1, bool fast, bool init) {
 // Transposed-output kernels exist for the two 5-slice-ring 3D shapes: 8 x 4
 // with 1-2 channels, 16 x 2 with 1 channel (2 channels would exceed the LDS).
 constexpr int kXstMaxC = 2;
-const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast, bool xst = false, bool init = false) {
+const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast, bool xst = false, bool init = false,
+                        bool comm = false) {
   if (xst && init) return nullptr;
+  if (comm) return (NW == 8 && R == 4 && P == 5) ? pick_comm_kernel(C, lamb1, fast, xst, init) : nullptr;
   if (xst) {
     if (NW == 8 && R == 4 && P == 5 && C == 1) return pick_variant<8, 4, 5, 1, true>(lamb1, fast);
     if (NW == 8 && R == 4 && P == 5 && C == 2) return pick_variant<8, 4, 5, 2, true>(lamb1, fast);
@@ -1358,6 +1517,10 @@ int coresident_groups(const void *kern, int threads, const Device &dev, long lon
   return FG_OK;
 }
 
+// Address of the exchange-wait histogram on device `dev` (the device is current).
+int exchange_histogram_impl(int dev, unsigned **out);
+inline int exchange_histogram(int dev, unsigned **out) { return exchange_histogram_impl(dev, out); }
+
 // Executable graphs of repeated launch sequences.  A replay whose sequence of
 // launches (kernel, grid and block of each, in order) repeats on a host
 // thread and device runs as one hipGraphLaunch of a cached graph whose kernel
@@ -1447,7 +1610,7 @@ struct Runner {
     pass_plane(x, 1, nA, nB, nE);
     if (nA <= 1 || nB <= 1) return false;
     const Plan p = plan_plane(nB, nE, K, x.C, dev.cus);
-    if (!p.ok || !pick_kernel(p.NW, p.R, p.P, x.C, x.lamb1, x.fast, true)) return false;
+    if (!p.ok || !pick_kernel(p.NW, p.R, p.P, x.C, x.lamb1, x.fast, true, false, p.comm)) return false;
     if ((long long)x.D * x.H * x.W * 4 >= 0xFFFFFFF0ll) return false;
     return slice_offsets_fit(x, 1);
   }
@@ -1480,7 +1643,7 @@ struct Runner {
   }
 
   int record_tiled(std::vector<SweepJob> &jobs, int dir, int NW, int R, int P, int C, bool lamb1, bool fast,
-                   bool xs = false, bool in = false) {
+                   bool xs = false, bool in = false, bool cm = false) {
     std::vector<SweepDesc> descs;
     long long halo_used = 0;
     int tiles = 0, max_blocks = 0, kk = -1;
@@ -1490,7 +1653,7 @@ struct Runner {
       fill_desc(*j.x, j.pass, dir, s, xs);
       alg += (double)(s.nA - 1) * s.nB * s.nE * (4.0 * s.C + 8.0);
       Plan p = plan_fixed(s.nB, s.nE, j.K, s.C);
-      if (!p.ok || p.NW != NW || p.R != R || p.P != P || p.K % P != 0 || (kk >= 0 && kk != p.K))
+      if (!p.ok || p.NW != NW || p.R != R || p.P != P || p.comm != cm || p.K % P != 0 || (kk >= 0 && kk != p.K))
         return fail(FG_EINVAL, "internal: sweep plan does not match the selected kernel");
       s.nTB = p.nTB; s.nTE = p.nTE; s.TB = p.TB; s.TE = p.TE;
       s.tile0 = tiles;
@@ -1510,10 +1673,10 @@ struct Runner {
     }
     Cmd c;
     c.type = Cmd::TILED;
-    c.kern = pick_kernel(NW, R, P, C, lamb1, fast, xs, in);
+    c.kern = pick_kernel(NW, R, P, C, lamb1, fast, xs, in, cm);
     if (!c.kern) return fail(FG_EINVAL, "no tiled kernel for this shape");
     c.grid = tiles;
-    c.block = NW * 64;
+    c.block = (NW + (cm ? kCommWaves : 0)) * 64;
     c.ndesc = (int)descs.size();
     if (c.ndesc == 1) {
       c.by_value = true;
@@ -1664,7 +1827,7 @@ struct Runner {
   int sweep_pass(std::vector<Xform *> &active, int pass) {
     const std::vector<int> ks = pass_halos(active, pass);
     for (int dir : {+1, -1}) {
-      struct G { int NW, R, P, K, C; bool l1, fa, xs, in; std::vector<SweepJob> jobs; };
+      struct G { int NW, R, P, K, C; bool l1, fa, xs, in, cm; std::vector<SweepJob> jobs; };
       std::vector<G> groups;
       for (size_t i = 0; i < active.size(); ++i) {
         Xform *x = active[i];
@@ -1682,15 +1845,15 @@ struct Runner {
         bool found = false;
         for (auto &g : groups)
           if (g.NW == p.NW && g.R == p.R && g.P == p.P && g.K == p.K && g.C == x->C && g.l1 == x->lamb1 &&
-              g.fa == x->fast && g.xs == xs && g.in == in) {
+              g.fa == x->fast && g.xs == xs && g.in == in && g.cm == p.comm) {
             g.jobs.push_back({x, pass, p.K});
             found = true;
             break;
           }
-        if (!found) groups.push_back({p.NW, p.R, p.P, p.K, x->C, x->lamb1, x->fast, xs, in, {{x, pass, p.K}}});
+        if (!found) groups.push_back({p.NW, p.R, p.P, p.K, x->C, x->lamb1, x->fast, xs, in, p.comm, {{x, pass, p.K}}});
       }
       for (auto &g : groups) {
-        int rc = record_group(g.jobs, dir, g.NW, g.R, g.P, g.C, g.l1, g.fa, g.xs, g.in);
+        int rc = record_group(g.jobs, dir, g.NW, g.R, g.P, g.C, g.l1, g.fa, g.xs, g.in, g.cm);
         if (rc) return rc;
       }
     }
@@ -1699,11 +1862,11 @@ struct Runner {
 
   // A group is split into launches whose tiles can all be co-resident.
   int record_group(std::vector<SweepJob> &jobs, int dir, int NW, int R, int P, int C, bool l1, bool fa,
-                   bool xs = false, bool in = false) {
-    const void *kern = pick_kernel(NW, R, P, C, l1, fa, xs, in);
+                   bool xs = false, bool in = false, bool cm = false) {
+    const void *kern = pick_kernel(NW, R, P, C, l1, fa, xs, in, cm);
     if (!kern) return fail(FG_EINVAL, "no tiled kernel for this shape");
     long long cap = 0;
-    int rc = coresident_groups(kern, NW * 64, dev, cap);
+    int rc = coresident_groups(kern, (NW + (cm ? kCommWaves : 0)) * 64, dev, cap);
     if (rc) return rc;
     if (max_groups > 0) cap = std::min<long long>(cap, max_groups);
     // A launch lasts as long as its longest sweep: pack planes of similar
@@ -1728,14 +1891,14 @@ struct Runner {
         continue;
       }
       if (tiles + nt > cap) {
-        if ((rc = record_tiled(chunk, dir, NW, R, P, C, l1, fa, xs, in))) return rc;
+        if ((rc = record_tiled(chunk, dir, NW, R, P, C, l1, fa, xs, in, cm))) return rc;
         chunk.clear();
         tiles = 0;
       }
       chunk.push_back(j);
       tiles += nt;
     }
-    if (!chunk.empty()) return record_tiled(chunk, dir, NW, R, P, C, l1, fa, xs, in);
+    if (!chunk.empty()) return record_tiled(chunk, dir, NW, R, P, C, l1, fa, xs, in, cm);
     return FG_OK;
   }
 
@@ -1792,10 +1955,10 @@ struct Runner {
     pass_plane(x, pass, nA, nB, nE);
     const Plan p = plan_fixed(nB, nE, K > 0 ? K : 1, x.C);
     if (!p.ok) return FG_OK;
-    const void *kern = pick_kernel(p.NW, p.R, p.P, x.C, x.lamb1, x.fast, false, true);
+    const void *kern = pick_kernel(p.NW, p.R, p.P, x.C, x.lamb1, x.fast, false, true, p.comm);
     if (!kern) return FG_OK;
     long long cap = 0;
-    const int rc = coresident_groups(kern, p.NW * 64, dev, cap);
+    const int rc = coresident_groups(kern, (p.NW + (p.comm ? kCommWaves : 0)) * 64, dev, cap);
     if (rc) return rc;
     if (max_groups > 0) cap = std::min<long long>(cap, max_groups);
     tiled = (long long)p.nTB * p.nTE <= cap;
@@ -1835,7 +1998,10 @@ struct Runner {
     };
     const size_t n = rec.cmds.size();
     std::vector<Vals> vals(n);
-    std::vector<std::array<void *, 8>> args(n);
+    std::vector<std::array<void *, 9>> args(n);
+    // exchange-wait histogram of this device while profiling
+    unsigned *xhist = nullptr;
+    if (g_prof_xhist && (rc = exchange_histogram(dev.id, &xhist))) return rc;
     std::vector<const void *> funcs(n);
     bool slice = false;
     for (size_t i = 0; i < n; ++i) {
@@ -1849,24 +2015,25 @@ struct Runner {
           v.k = c.K;
           v.sp = spin;
           v.tb = c.tag_base;
-          a = {(void *)&v.dp, &v.ndesc, (void *)&c.sweep1, &v.k, &v.tb, (void *)&halo, (void *)&status, &v.sp};
+          a = {(void *)&v.dp, &v.ndesc, (void *)&c.sweep1, &v.k, &v.tb, (void *)&halo, (void *)&status, &v.sp,
+               (void *)&xhist};
           funcs[i] = c.kern;
           break;
         case Cmd::SLICE:
           v.a = c.a;
           v.aq = c.aq;
-          a = {(void *)&c.slice, &v.a, &v.aq, nullptr, nullptr, nullptr, nullptr, nullptr};
+          a = {(void *)&c.slice, &v.a, &v.aq, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
           funcs[i] = reinterpret_cast<const void *>(&slice_kernel);
           slice = true;
           break;
         case Cmd::XPOSE:
           v.dp = c.by_value ? nullptr : blob_dev + c.off;
-          a = {(void *)&v.dp, &v.ndesc, (void *)&c.xpose1, nullptr, nullptr, nullptr, nullptr, nullptr};
+          a = {(void *)&v.dp, &v.ndesc, (void *)&c.xpose1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
           funcs[i] = reinterpret_cast<const void *>(&xpose_kernel);
           break;
         case Cmd::EW:
           v.dp = c.by_value ? nullptr : blob_dev + c.off;
-          a = {(void *)&v.dp, &v.ndesc, (void *)&c.ew1, nullptr, nullptr, nullptr, nullptr, nullptr};
+          a = {(void *)&v.dp, &v.ndesc, (void *)&c.ew1, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
           funcs[i] = reinterpret_cast<const void *>(&ew_kernel);
           break;
       }
@@ -1884,7 +2051,7 @@ struct Runner {
 
     // graph replay of a repeated sequence (not while profiling per launch, not
     // for the per-slice engine's hundreds of launches, never without allocating)
-    if (g_graph_on && !g_prof_on && !slice && !no_alloc && n >= 2) {
+    if (g_graph_on && !g_prof_on && !g_prof_xhist && !slice && !no_alloc && n >= 2) {
       uint64_t key = 1469598103934665603ull;
       auto mix = [&](uint64_t x) { key = (key ^ x) * 1099511628211ull; };
       for (size_t i = 0; i < n; ++i) {
@@ -2337,7 +2504,26 @@ int run_single(OpSpec o, void *ws, size_t wsb, void *stream) {
 
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
 
+// Exchange-wait histogram filled by the sweep kernels while profiling.
+static __device__ unsigned xhist_dev[kXHistBuckets];
+
 namespace {
+
+std::mutex g_xhist_m;
+std::map<int, unsigned *> g_xhist_ptr;  // device -> address of xhist_dev
+
+int exchange_histogram_impl(int dev, unsigned **out) {
+  std::lock_guard<std::mutex> g(g_xhist_m);
+  auto it = g_xhist_ptr.find(dev);
+  if (it == g_xhist_ptr.end()) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(xhist_dev));
+    if (e != hipSuccess || !p) return fail(FG_EHIP, std::string("hipGetSymbolAddress(xhist): ") + hipGetErrorString(e));
+    it = g_xhist_ptr.emplace(dev, (unsigned *)p).first;
+  }
+  *out = it->second;
+  return FG_OK;
+}
 
 // Device address of this device's scratch array (the device is current),
 // looked up once per device.
@@ -2503,7 +2689,7 @@ int fg_tuning_set(const fg_tuning *t) {
   if (t->engine < 0 || t->engine > 1) return fail(FG_EINVAL, "engine must be 0 or 1");
   if (t->halo < 0 || t->halo > kMaxK) return fail(FG_EINVAL, "halo must be 0 (automatic) or in [1, 24]");
   if (t->max_groups < 0) return fail(FG_EINVAL, "max_groups must be >= 0");
-  if (t->shape < 0 || t->shape >= kNumShapes) return fail(FG_EINVAL, "shape must be in [0, 5]");
+  if (t->shape < 0 || t->shape >= kNumShapes) return fail(FG_EINVAL, "shape must be in [0, 6]");
   g_tune = *t;
   g_shape = t->shape == 0 ? kAutoShape : t->shape;
   return FG_OK;
@@ -2516,7 +2702,32 @@ int fg_tuning_get(fg_tuning *t) {
 }
 
 int fg_profile_enable(int on) {
-  g_prof_on = on != 0;
+  g_prof_on = (on & 1) != 0;
+  g_prof_xhist = (on & 2) != 0;
+  if (g_prof_xhist) {  // start a new exchange-wait histogram on the current device
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(FG_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    unsigned *h = nullptr;
+    const int rc = exchange_histogram_impl(dev, &h);
+    if (rc) return rc;
+    e = hipMemset(h, 0, kXHistBuckets * sizeof(unsigned));
+    if (e != hipSuccess) return fail(FG_EHIP, std::string("hipMemset: ") + hipGetErrorString(e));
+  }
+  return FG_OK;
+}
+
+int fg_profile_exchange_histogram(unsigned *counts, int n) {
+  if (!counts || n < kXHistBuckets) return fail(FG_EINVAL, "need room for 32 buckets");
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(FG_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  unsigned *h = nullptr;
+  const int rc = exchange_histogram_impl(dev, &h);
+  if (rc) return rc;
+  e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(counts, h, kXHistBuckets * sizeof(unsigned), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail(FG_EHIP, std::string("histogram copy: ") + hipGetErrorString(e));
   return FG_OK;
 }
 

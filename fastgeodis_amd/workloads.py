@@ -112,14 +112,21 @@ def run_c4(device="cuda", reps: int = 3):
             return generalised_geodesic2d(i, m, 1e10, 1.0, 2)
         return generalised_geodesic3d(i, m, sp, 1e10, 1.0, 2)
 
+    from . import autotune
+
     t_loop, outs_loop = _timed(lambda: [single(*it) for it in items], reps)
     insts = [Instance(i, m, 1e10, 1.0, iter=2, spacing=sp) for i, m, sp in items]
     t_batch, outs_batch = _timed(lambda: geodesic_batch(insts), reps)
-    same = all(_same(a, b) for a, b in zip(outs_loop, outs_batch))
+    tuning = autotune.tune_batch(insts, reps=max(1, reps - 1))
+    t_tuned, outs_tuned = _timed(lambda: geodesic_batch(insts, tuned=True), reps)
+    same = all(_same(a, b) for a, b in zip(outs_loop, outs_batch)) and \
+        all(_same(a, b) for a, b in zip(outs_batch, outs_tuned))
     return {"workload": "c4 256 uneven 2D (64-2048, C 1|3) + 32 uneven 3D (64-256), lamb 1, iter 2",
             "identical": same, "voxel_updates": n_upd,
-            "loop_ms": t_loop * 1e3, "batch_ms": t_batch * 1e3,
-            "loop_vups": n_upd / t_loop, "batch_vups": n_upd / t_batch, "speedup": t_loop / t_batch}
+            "loop_ms": t_loop * 1e3, "batch_ms": t_batch * 1e3, "tuned_batch_ms": t_tuned * 1e3,
+            "tuning": tuning["best"], "tuning_table": tuning["table"],
+            "loop_vups": n_upd / t_loop, "batch_vups": n_upd / t_batch, "tuned_vups": n_upd / t_tuned,
+            "speedup": t_loop / t_batch, "tuned_speedup": t_loop / t_tuned}
 
 
 def run_c5(rates=(50, 200, 500, 1000, 2000), requests: int = 1000, sessions: int = 64,

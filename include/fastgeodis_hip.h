@@ -200,7 +200,9 @@ typedef struct fg_tuning {
   int halo;
   int max_groups; /* cap on persistent workgroups (0 = occupancy limit) */
   int shape;      /* 3D plane workgroup shape (waves x rows / slab ring): 0 auto
-                     (= 5), 1 = 4x8/2, 2 = 8x4/3, 3 = 16x2/3, 4 = 8x4/5, 5 = 16x2/5 */
+                     (= 5), 1 = 4x8/2, 2 = 8x4/3, 3 = 16x2/3, 4 = 8x4/5, 5 = 16x2/5,
+                     6 = 8x4/5 plus four waves that perform the halo exchanges
+                     (1-2 channels; 3-4 channels run 8x4/2) */
 } fg_tuning;
 int fg_tuning_set(const fg_tuning *t);
 int fg_tuning_get(fg_tuning *t);
@@ -215,12 +217,27 @@ int fg_set_graphs(int on);
 
 /* Per-launch timing of the persistent sweep kernels, for measurement tools
  * (no counterpart in the reference).  While enabled on a host thread, every
- * tiled sweep launch it issues is bracketed by HIP events on its stream.
+ * tiled sweep launch it issues is bracketed by HIP events on its stream
+ * (fg_profile_enable(1); 2 records the exchange-wait histogram below instead,
+ * 3 both; 0 off).
  * fg_profile_collect waits for those events and returns the number of
  * launches, their summed duration (ms) and their summed algorithmic bytes
  * ((4*C + 8) bytes per voxel update), then clears the record. */
 int fg_profile_enable(int on);
 int fg_profile_collect(int *launches, double *total_ms, double *alg_bytes);
+
+/* While the histogram is enabled (fg_profile_enable(2)), every wave of every
+ * tiled sweep records
+ * how long each of its halo exchanges took (from its publications to the
+ * last neighbour value it waited for) on the device's constant-rate clock
+ * (hipDeviceAttributeWallClockRate kHz), as a histogram of 32 log2 buckets:
+ * bucket b counts exchanges of [2^(b-1), 2^b) ticks, bucket 0 of 0 ticks.
+ * Each workgroup counts in its LDS and adds its counts to the device's
+ * histogram once, at its end.  fg_profile_enable(2) clears the current
+ * device's histogram;
+ * fg_profile_exchange_histogram synchronises the device and copies it into
+ * counts[0..31] (n >= 32). */
+int fg_profile_exchange_histogram(unsigned *counts, int n);
 
 #ifdef __cplusplus
 }

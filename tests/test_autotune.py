@@ -75,3 +75,44 @@ def test_tuned_choice_matches_the_oracle(device, monkeypatch):
         FG.set_tuning(**old)
     want = oracle.py_geodesic3d(img, mask, [1.5, 1.0, 0.8], 1e10, 0.5, 2)
     assert np.array_equal(out.view(np.uint32), want.view(np.uint32))
+
+
+def test_batch_signature_is_a_size_class():
+    """Extents rounded to powers of two, order-independent, op and variant aware."""
+    a = FG.Instance(torch.zeros(1, 1, 100, 200), torch.ones(1, 1, 100, 200), 1e10, 1.0)
+    b = FG.Instance(torch.zeros(1, 1, 96, 210), torch.ones(1, 1, 96, 210), 1e10, 1.0)
+    c = FG.Instance(torch.zeros(1, 3, 30, 40, 50), torch.ones(1, 1, 30, 40, 50), 1e10, 0.5, op="signed")
+    assert autotune.batch_signature([a, c]) == autotune.batch_signature([c, b])
+    assert autotune.batch_signature([a, c]) != autotune.batch_signature([a, a])
+    d = FG.Instance(torch.zeros(1, 3, 30, 40, 50), torch.ones(1, 1, 30, 40, 50), 1e10, 1.0, op="signed")
+    assert autotune.batch_signature([a, c]) != autotune.batch_signature([a, d])
+
+
+@pytest.mark.gpu
+def test_tuned_batch_equals_analytic_and_uses_the_cache(device, monkeypatch, tmp_path):
+    import numpy as np
+
+    monkeypatch.setenv("FASTGEODIS_AMD_TUNE_CACHE", str(tmp_path / "t.json"))
+    autotune._cache.clear()
+    rng = np.random.default_rng(5)
+    insts = []
+    for k in range(5):
+        shape = (int(rng.integers(20, 90)), int(rng.integers(20, 130))) if k % 2 else \
+            tuple(int(x) for x in rng.integers(10, 50, size=3))
+        img = torch.from_numpy(rng.random((1, 1, *shape), dtype=np.float32)).cuda()
+        m = torch.ones((1, 1, *shape), device="cuda")
+        m[(0, 0) + tuple(int(rng.integers(0, s)) for s in shape)] = 0
+        insts.append(FG.Instance(img, m, 1e10, 1.0, iter=2, spacing=[1.5, 1.0, 1.0]))
+    old = FG.get_tuning()
+    res = autotune.tune_batch(insts, reps=1)
+    assert FG.get_tuning() == old
+    assert len(res["table"]) == len(autotune.DEFAULT_BATCH_CANDIDATES)
+    assert (tmp_path / "t.json").exists()
+    a = FG.geodesic_batch(insts)
+    b = FG.geodesic_batch(insts, tuned=True)
+    assert FG.get_tuning() == old
+    for x, y in zip(a, b):
+        assert torch.equal(x.view(torch.int32), y.view(torch.int32))
+    autotune._cache.clear()
+    assert autotune.tuning_for_batch(insts, measure=False) == res["best"]  # reloaded from the file
+    autotune._cache.clear()

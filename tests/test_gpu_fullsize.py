@@ -65,7 +65,8 @@ def test_headline_512_engines_and_tilings_agree(headline):
     call = lambda: FG.generalised_geodesic3d(img, mask, [1.0, 1.0, 1.0], 1e10, 1.0, 2)  # noqa: E731
     say("[fullsize] 512^3 iter 2: default tiling")
     base = call().cpu().numpy()
-    for engine, halo, shape, name in [(1, 10, 0, "tiled K=10"), (1, 0, 2, "tiled 8x4/3"), (0, 0, 0, "per-slice")]:
+    for engine, halo, shape, name in [(1, 10, 0, "tiled K=10"), (1, 0, 2, "tiled 8x4/3"), (1, 0, 6, "tiled 8x4 exchange waves"),
+                                      (0, 0, 0, "per-slice")]:
         say(f"[fullsize] 512^3 iter 2: {name}")
         other = with_tuning(engine, halo, shape, call).cpu().numpy()
         assert bits_equal(base, other), name
