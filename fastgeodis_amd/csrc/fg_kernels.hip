@@ -65,10 +65,12 @@ constexpr float kInf = __builtin_huge_valf();
 // Deepest halo: 2D rows use up to 24 of a 64-lane window (one-wave tiles are
 // bound by the exchange, and a lone image leaves most CUs idle); 3D windows
 // of 32 rows stop at 15 by themselves.
-constexpr int kMaxK = 24;
+constexpr int kMaxK = 30;
 // s_sleep units (64 clocks each) between publishing a halo and first polling
 // the neighbours' publications.
 constexpr int kHaloPollDelay = 24;
+// The same for one-wave row tiles (two neighbours each): 512 clocks.
+constexpr int kRowPollDelay = 8;
 // Waves that perform the halo exchanges in kernels with exchange waves.
 constexpr int kCommWaves = 4;
 // Buckets of the exchange-wait histogram: bucket b counts waits of
@@ -223,6 +225,28 @@ __device__ __forceinline__ float relax(const float (&dq)[3], const float (&Iq)[3
   }
 }
 
+// New value of an element of a one-row plane: the three candidates of the
+// front row (lanes -1, 0, +1; table entries 3..5).  The six candidates of
+// rows -1 and +1, which lie outside such a plane, are skipped as the
+// reference skips out-of-range neighbours.
+template <int CT, bool LAMB1, bool FAST>
+__device__ __forceinline__ float relax_row(float dq, const float (&Iq)[CT], const float (&In)[CT], float dinit,
+                                           const float (&e1)[9], float lg) {
+  const float c3 = candidate<1, 0, CT, LAMB1>(dq, Iq, In, e1[3], lg);
+  const float c4 = candidate<1, 1, CT, LAMB1>(dq, Iq, In, e1[4], lg);
+  const float c5 = candidate<1, 2, CT, LAMB1>(dq, Iq, In, e1[5], lg);
+  if constexpr (FAST) {
+    const float m = fminf(fminf(c3, c4), c5);
+    return (m < dinit) ? m : dinit;
+  } else {
+    float nd = dinit;
+    nd = (c3 < nd) ? c3 : nd;
+    nd = (c4 < nd) ? c4 : nd;
+    nd = (c5 < nd) ? c5 : nd;
+    return nd;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Tiled persistent sweep kernel
 // ---------------------------------------------------------------------------
@@ -265,6 +289,9 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   __shared__ float edgeI_first[P][NWL][CT][64];
   __shared__ float edgeI_last[P][NWL][CT][64];
   constexpr int NWR = NW * R;  // window rows
+  // one wave of one row: the plane is a single row (nB == 1), so the only
+  // neighbours are lanes -1, 0, +1 of the front row itself
+  constexpr bool ROW1 = NW == 1 && R == 1;
   static_assert(!XST || (NW > 1 && 64 % NWR == 0), "transposed output needs a multi-wave window of 16/32/64 rows");
   __shared__ float xtile[XST ? P : 1][XST ? NWR : 1][65];
   __shared__ unsigned xhist_wg[kXHistBuckets];  // this workgroup's exchange-wait counts (measurement)
@@ -324,7 +351,7 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   // The block loop below advances in groups of P steps and exchanges halos
   // every K steps; it stays within the sweep only when K is a positive
   // multiple of P.  A launch that breaks this is refused before any store.
-  if (K <= 0 || K % P != 0) {
+  if (K <= 0 || K % P != 0 || (ROW1 && Dd.nB != 1)) {
     if (threadIdx.x == 0) __hip_atomic_store(status, FG_EINVAL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     return;
   }
@@ -429,11 +456,56 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     rvalid[r] = rb[r] >= 0 && rb[r] < nB;
     rofs[r] = (unsigned)((long long)min(max(rb[r], 0), nB - 1) * sB * 4) + ecb;
   }
-  // buffer resource of slice a of an array (raw, byte-addressed); offsets at
-  // or beyond the range are out of bounds and stores there are dropped
-  auto slice_rsrc = [&](const float *base, int a, unsigned range = 0xFFFFFFF0u) {
-    return __builtin_amdgcn_make_buffer_rsrc((void *)(base + (long long)a * sA), 0, range, 0x00020000);
+  // Buffer resources are raw and byte-addressed; offsets at or beyond the
+  // range are out of bounds, and stores there are dropped.
+  // Slice cursor of a step: its slice a and that slice's byte offset
+  // a * sA * 4, advanced together by one slice per step (two scalar adds
+  // instead of a 64-bit multiply per array).  A slice outside [0, nA) (the
+  // prefetches and stores of steps past the end of the sweep) gets an empty
+  // buffer range: its loads return 0 and its stores are dropped, without a
+  // memory access.
+  // FLAT (the one-row kernels, whose steps are latency-bound): one buffer
+  // resource per array for the whole sweep, with the array as its range
+  // (nA * sA * 4 bytes), and 32-bit offsets a * sA * 4 + lane offset, so a
+  // step costs two vector adds instead of scalar resource arithmetic.  The
+  // host runs these kernels only when (nA + 32) * sA * 4 <= 2^31: then the
+  // offsets of slices just past either end (a in [-P, nA + P), P <= 32) wrap
+  // to or stay at or beyond the range, and so do the store offsets of lanes
+  // that store nothing, which are based at 2^31.
+  constexpr bool FLAT = ROW1;
+  struct Cursor {
+    int a;
+    long long off;
+    unsigned foff;  // FLAT: off modulo 2^32
   };
+  const long long dAb = (long long)dir * sA * 4;
+  auto cursor_of = [&](int step) {
+    const int a = a_first + step * dir;
+    const long long off = (long long)a * sA * 4;
+    return Cursor{a, off, (unsigned)off};
+  };
+  auto advance = [&](Cursor &c) {
+    if constexpr (FLAT) {
+      c.foff += (unsigned)dAb;
+    } else {
+      c.a += dir;
+      c.off += dAb;
+    }
+  };
+  const unsigned flat_range = FLAT ? (unsigned)((long long)nA * sA * 4) : 0u;
+  auto cursor_rsrc = [&](const float *base, const Cursor &c) {
+    if constexpr (FLAT) {
+      return __builtin_amdgcn_make_buffer_rsrc((void *)base, 0, flat_range, 0x00020000);
+    } else {
+      const bool in = (unsigned)c.a < (unsigned)nA;
+      return __builtin_amdgcn_make_buffer_rsrc((void *)((const char *)base + c.off), 0, in ? 0xFFFFFFF0u : 0u,
+                                               0x00020000);
+    }
+  };
+  // byte offset of a lane's element within its resource; the out-of-range
+  // offset of a lane that stores nothing
+  auto at = [&](unsigned lofs, const Cursor &c) { return FLAT ? lofs + c.foff : lofs; };
+  auto nowhere = [&](const Cursor &c) { return FLAT ? 0x80000000u + c.foff : 0xFFFFFFFFu; };
 
   // Transposed output: wave w moves window columns (w * R + k) * (64 / NWR)
   // + lane / NWR, rows lane % NWR, for k < R; the byte offset of each element
@@ -486,25 +558,26 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
       return m;
     }
   };
-  auto load_rows = [&](Slab<R, CT> &S, int a, int r0, int r1) {
-    const __amdgpu_buffer_rsrc_t rd = slice_rsrc(INIT ? Dd.init_src : Dd.d, a);
+  auto load_rows = [&](Slab<R, CT> &S, const Cursor &cur, int r0, int r1) {
+    const __amdgpu_buffer_rsrc_t rd = cursor_rsrc(INIT ? Dd.init_src : Dd.d, cur);
     __amdgpu_buffer_rsrc_t ri[CT];
 #pragma unroll
-    for (int c = 0; c < CT; ++c) ri[c] = slice_rsrc(Dd.img + c * cs, a);
+    for (int c = 0; c < CT; ++c) ri[c] = cursor_rsrc(Dd.img + c * cs, cur);
 #pragma unroll
     for (int r = 0; r < R + 2; ++r) {
-      if (r >= r0 && r < r1) {
+      // the one-row kernel (a plane of one row) uses only its own image row
+      if (r >= r0 && r < r1 && (!ROW1 || r == 1)) {
 #pragma unroll
         for (int c = 0; c < CT; ++c)
-          S.I[c][r] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ri[c], (int)rofs[r], 0, 0));
+          S.I[c][r] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ri[c], (int)at(rofs[r], cur), 0, 0));
       }
       if (r >= 1 && r <= R)
-        S.d[r - 1] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rd, (int)rofs[r], 0, 0));
+        S.d[r - 1] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rd, (int)at(rofs[r], cur), 0, 0));
     }
   };
-  auto load_slab = [&](Slab<R, CT> &S, int a) {
-    if constexpr (NW > 1) load_rows(S, a, 1, R + 1);
-    else load_rows(S, a, 0, R + 2);
+  auto load_slab = [&](Slab<R, CT> &S, const Cursor &cur) {
+    if constexpr (NW > 1) load_rows(S, cur, 1, R + 1);
+    else load_rows(S, cur, 0, R + 2);
   };
 
   // per-row validity of this lane (pad positions hold +inf in the front) and
@@ -522,15 +595,16 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   // S[P - 1].  Slabs stay in fixed registers this way.
   float Fd[R];  // front distance rows 0..R-1
   Slab<R, CT> S[P];
-  load_rows(S[P - 1], a_first - dir, 0, R + 2);  // front slice: not modified by this sweep
+  load_rows(S[P - 1], cursor_of(-1), 0, R + 2);  // front slice: not modified by this sweep
 #pragma unroll
   for (int r = 0; r < R; ++r) Fd[r] = pvalid[r] ? init_of(S[P - 1].d[r]) : kInf;
   if constexpr (INIT) {
     // the output's copy of the untouched first slice: its initialisation
-    const __amdgpu_buffer_rsrc_t rs = slice_rsrc(Dd.d, a_first - dir);
+    const Cursor c0 = cursor_of(-1);
+    const __amdgpu_buffer_rsrc_t rs = cursor_rsrc(Dd.d, c0);
 #pragma unroll
     for (int r = 0; r < R; ++r) {
-      const unsigned off = (row_own[r] && lane_own) ? rofs[r + 1] : 0xFFFFFFFFu;
+      const unsigned off = (row_own[r] && lane_own) ? at(rofs[r + 1], c0) : nowhere(c0);
       __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(Fd[r]), rs, (int)off, 0, 0);
     }
   }
@@ -539,7 +613,9 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     for (int r = 0; r < R; ++r) xtile[P - 1][w * R + r][lane] = Fd[r];
   }
 #pragma unroll
-  for (int p = 0; p < P - 1; ++p) load_slab(S[p], min(max(a_first + p * dir, 0), nA - 1));
+  for (int p = 0; p < P - 1; ++p) load_slab(S[p], cursor_of(p));
+  // the slices that the loop stores (step s) and prefetches (step s + P - 1)
+  Cursor cur_st = cursor_of(0), cur_pf = cursor_of(P - 1);
 
   // Publishes the tile's own front values that neighbours read and reloads
   // the halo positions of the window from the neighbours' publications.
@@ -553,11 +629,12 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   // memory round trip serves all of them.
   bool wd_tripped = false;
   // COMM: hand the front to the exchange wave and take the halo back from it
-  auto exchange_via_comm = [&](int blk, Slab<R, CT> &pf_slab, int pf_step) {
+  auto exchange_via_comm = [&](int blk, Slab<R, CT> &pf_slab) {
     (void)blk;
 #pragma unroll
     for (int r = 0; r < R; ++r) xfront[w * R + r][lane] = Fd[r];
-    load_slab(pf_slab, min(max(a_first + pf_step * dir, 0), nA - 1));
+    load_slab(pf_slab, cur_pf);
+    advance(cur_pf);
     __syncthreads();  // front visible to the exchange wave
     __syncthreads();  // halo values written by it
     const int lane_dist = max(max(e_own0 - e, e - (e_own1 - 1)), 0);
@@ -570,7 +647,7 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
       if (need) Fd[r] = xfront[w * R + r][lane];
     }
   };
-  auto exchange = [&](int blk, Slab<R, CT> &pf_slab, int pf_step) {
+  auto exchange = [&](int blk, Slab<R, CT> &pf_slab) {
     // measurement only (xhist non-null): how long this wave spends in the
     // exchange, from its publications to the last halo value it waits for,
     // as a histogram of the device's constant-rate clock (log2 buckets)
@@ -591,8 +668,10 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     }
     // Neighbours publish at about the same moment; polling right away mostly
     // finds their values not yet visible and costs a second round trip.
-    // Waiting ~1500 clocks first was measured fastest (16..24 x 64 clocks).
-    __builtin_amdgcn_s_sleep(kHaloPollDelay);
+    // Waiting ~1500 clocks first was measured fastest for multi-wave tiles
+    // (16..24 x 64 clocks), 512 clocks for one-wave row tiles.
+    if constexpr (NW == 1) __builtin_amdgcn_s_sleep(kRowPollDelay);
+    else __builtin_amdgcn_s_sleep(kHaloPollDelay);
     // Only lanes that need a halo value load (measured faster than letting
     // every lane load some in-bounds line and selecting afterwards).
     const gu64 *src[R];
@@ -612,7 +691,8 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
       x[r] = 0;
       if (need[r]) x[r] = __hip_atomic_load(src[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    load_slab(pf_slab, min(max(a_first + pf_step * dir, 0), nA - 1));
+    load_slab(pf_slab, cur_pf);
+    advance(cur_pf);
     // after an expiry this wave waits no more: the sweep then finishes
     // (with wrong values, reported through the status word) instead of
     // spending the full limit again at every later exchange
@@ -646,9 +726,8 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   // Afterwards Fr is refilled with the slice of step s + P - 1.
   auto step = [&](Slab<R, CT> &Sl, Slab<R, CT> &Fr, int s, int slot, bool prefetch) {
     // steps past the end of the sweep (the last block is padded to K steps)
-    // compute on clamped slices and store nothing
-    const bool live = s < nsteps;
-    const int a = min(max(a_first + s * dir, 0), nA - 1);
+    // compute on slices of zeros and store nothing
+    (void)s;
     // wave-boundary front rows through LDS, and the new slice's image rows
     // -1 and R (the front image of the next step) from the neighbour waves
     // (branch-free: the edge waves read their own slot and select +inf for
@@ -676,8 +755,18 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
       xflush((slot + P - 1) % P, s - 1);
     }
     float nd[R];
+    if constexpr (ROW1) {
+      float Iq[CT], In[CT];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
+      for (int c = 0; c < CT; ++c) {
+        Iq[c] = Fr.I[c][1];
+        In[c] = Sl.I[c][1];
+      }
+      const float v = relax_row<CT, LAMB1, FAST>(Fd[0], Iq, In, init_of(Sl.d[0]), e1, lg);
+      nd[0] = pvalid[0] ? v : kInf;
+    }
+#pragma unroll
+    for (int r = 0; r < (ROW1 ? 0 : R); ++r) {
       float dq[3];
       dq[0] = (r == 0) ? top : Fd[r - 1];
       dq[1] = Fd[r];
@@ -699,19 +788,23 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
 #pragma unroll
       for (int r = 0; r < R; ++r) xtile[slot][w * R + r][lane] = nd[r];
     } else {
-      // a padded step's resource has an empty range: all its stores are dropped
-      const __amdgpu_buffer_rsrc_t rs = slice_rsrc(Dd.d, a, live ? 0xFFFFFFF0u : 0u);
+      // a padded step's slice is outside the array: all its stores are dropped
+      const __amdgpu_buffer_rsrc_t rs = cursor_rsrc(Dd.d, cur_st);
 #pragma unroll
       for (int r = 0; r < R; ++r) {
-        const unsigned off = (row_own[r] && lane_own) ? rofs[r + 1] : 0xFFFFFFFFu;
+        const unsigned off = (row_own[r] && lane_own) ? at(rofs[r + 1], cur_st) : nowhere(cur_st);
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(nd[r]), rs, (int)off, 0, 0);
       }
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) Fd[r] = nd[r];
-    // prefetch without a branch: near the end of the sweep the slice index is
-    // clamped and the (valid, unused) data is simply never consumed
-    if (prefetch) load_slab(Fr, min(max(a + (P - 1) * dir, 0), nA - 1));
+    advance(cur_st);
+    // prefetch without a branch: past the end of the sweep the loads return 0
+    // and the data is never consumed (only steps that store nothing use it)
+    if (prefetch) {
+      load_slab(Fr, cur_pf);
+      advance(cur_pf);
+    }
   };
 
   // Blocks of exactly K steps (K a multiple of P, so a block starts at a
@@ -734,8 +827,8 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     step(S[P - 1], S[P - 2], sl + P - 1, P - 1, false);
     if (blk + 1 >= nblocks) break;
     // deferred prefetch of step s0 + K - 1: the slice of step s0 + K + P - 2
-    if constexpr (COMM) exchange_via_comm(blk + 1, S[P - 2], s0 + K + P - 2);
-    else exchange(blk + 1, S[P - 2], s0 + K + P - 2);
+    if constexpr (COMM) exchange_via_comm(blk + 1, S[P - 2]);
+    else exchange(blk + 1, S[P - 2]);
   }
   if constexpr (XST) {
     __syncthreads();
@@ -1138,17 +1231,22 @@ struct Shape {
 };
 // Workgroup shapes with compiled kernels: index 0 serves 2D row sweeps
 // (one row); the others tile 3D planes (selected by fg_tuning.shape).
-constexpr Shape kShapes[] = {{1, 1, 8, 8, false},  {4, 8, 2, 2, false},  {8, 4, 3, 2, false},
+constexpr Shape kShapes[] = {{1, 1, 6, 6, false},  {4, 8, 2, 2, false},  {8, 4, 3, 2, false},
                              {16, 2, 3, 3, false}, {8, 4, 5, 2, false},  {16, 2, 5, 3, false},
                              {8, 4, 5, 2, true}};
 constexpr int kNumShapes = 7;
+// One-row planes at K = 30 (a lone row, or rows few enough for one tile per
+// CU at that depth): the whole block in the slab ring, so the prefetches of
+// a block are never waited for at a loop head.
+constexpr Shape kRowDeep = {1, 1, 30, 30, false};
+constexpr int kRowDeepK = 30;
 constexpr int kAutoShape = 5;  // fastest on 512^2 planes: 16 x 2 with the deepest ring that keeps K = 5
 thread_local int g_shape = kAutoShape;
 
 Plan plan_fixed(int nB, int nE, int Kreq, int C) {
   Plan p{};
   p.ok = false;
-  const Shape s = (nB == 1) ? kShapes[0] : kShapes[g_shape];
+  const Shape s = (nB == 1) ? (Kreq > kRowDeepK - kShapes[0].P12 ? kRowDeep : kShapes[0]) : kShapes[g_shape];
   p.NW = s.NW;
   p.R = s.R;
   p.comm = s.comm && C <= 2;  // exchange-wave kernels exist for 1-2 channels; 3-4 run the plain 8 x 4
@@ -1274,7 +1372,8 @@ const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast, bool
   }
 #define FG_S(nw, r, p, clo, chi) \
   if (NW == nw && R == r && P == p && C >= clo && C <= chi) return pick_kernel_nr<nw, r, p, clo, chi>(C, lamb1, fast, init);
-  FG_S(1, 1, 8, 1, 4)
+  FG_S(1, 1, 6, 1, 4)
+  FG_S(1, 1, 30, 1, 4)
   FG_S(4, 8, 2, 1, 4)
   FG_S(8, 4, 3, 1, 2)
   FG_S(8, 4, 5, 1, 2)
@@ -1616,11 +1715,14 @@ struct Runner {
   }
 
   // The tiled kernel addresses elements of a slice with 32-bit byte offsets
-  // from the slice start (buffer resources with a range of 0xFFFFFFF0 bytes).
+  // from the slice start (buffer resources with a range of 0xFFFFFFF0 bytes);
+  // the one-row kernel addresses the whole array with them, within 2^31
+  // bytes including 32 slices of margin past its end.
   bool slice_offsets_fit(const Xform &x, int pass) {
     SweepDesc s;
     fill_desc(x, pass, 1, s);
     const long long span = ((long long)(s.nB - 1) * s.sB + s.nE) * 4;
+    if (s.nB == 1 && (long long)(s.nA + 32) * s.sA * 4 > (1ll << 31)) return false;
     return span < 0xFFFFFFF0ll;
   }
 
@@ -2687,7 +2789,7 @@ int fg_set_voxels_f32(const fg_voxel_set *items, int n, void *stream) {
 int fg_tuning_set(const fg_tuning *t) {
   if (!t) return fail(FG_EINVAL, "null tuning");
   if (t->engine < 0 || t->engine > 1) return fail(FG_EINVAL, "engine must be 0 or 1");
-  if (t->halo < 0 || t->halo > kMaxK) return fail(FG_EINVAL, "halo must be 0 (automatic) or in [1, 24]");
+  if (t->halo < 0 || t->halo > kMaxK) return fail(FG_EINVAL, "halo must be 0 (automatic) or in [1, 30]");
   if (t->max_groups < 0) return fail(FG_EINVAL, "max_groups must be >= 0");
   if (t->shape < 0 || t->shape >= kNumShapes) return fail(FG_EINVAL, "shape must be in [0, 6]");
   g_tune = *t;

@@ -189,7 +189,7 @@ int fg_set_voxels_f32(const fg_voxel_set *items, int n, void *stream);
 
 /* Tile/launch parameters of the persistent sweep engine.  engine 0 is the
  * one-launch-per-slice engine, 1 the persistent tiled engine (default).
- * halo is the trapezoid depth K (slices per neighbour exchange, 1..24),
+ * halo is the trapezoid depth K (slices per neighbour exchange, 1..30),
  * rounded up to a multiple of the kernel's slab ring; 0 (default) picks the
  * deepest halo whose tiles fit one per compute unit: per 2D row and per lone
  * 3D plane, and over all 3D planes that share a pass (batches, the halves of

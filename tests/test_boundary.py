@@ -103,11 +103,11 @@ def test_auto_halo_workspace_covers_fixed_depths():
     old = FG.get_tuning()
     try:
         sizes = {}
-        for k in range(0, 25):
+        for k in range(0, 31):
             FG.set_tuning(1, k, 0, 0)
             sizes[k] = (FG.FastGeodisCpp.workspace_bytes(0, 1, 96, 300, 200),
                         FG.FastGeodisCpp.workspace_bytes(0, 3, 0, 700, 1500))
-        assert all(sizes[0][j] >= sizes[k][j] for k in range(1, 25) for j in range(2))
+        assert all(sizes[0][j] >= sizes[k][j] for k in range(1, 31) for j in range(2))
     finally:
         FG.set_tuning(**old)
 
@@ -125,10 +125,10 @@ def test_tuning_roundtrip():
         assert FG.get_tuning()["halo"] == 0
         with pytest.raises(ValueError):
             FG.set_tuning(1, -1, 0)
-        FG.set_tuning(1, 24, 0)  # deepest halo
-        assert FG.get_tuning()["halo"] == 24
+        FG.set_tuning(1, 30, 0)  # deepest halo
+        assert FG.get_tuning()["halo"] == 30
         with pytest.raises(ValueError):
-            FG.set_tuning(1, 25, 0)
+            FG.set_tuning(1, 31, 0)
     finally:
         FG.set_tuning(**old)
 
