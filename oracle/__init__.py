@@ -66,6 +66,10 @@ def _load():
     lib.fgo_geodesic3d_layout.argtypes = [p, p, p, i, i, i, i, i, p, f, f, f, i]
     lib.fgo_geodesic2d_layout.restype = ctypes.c_int
     lib.fgo_geodesic3d_layout.restype = ctypes.c_int
+    lib.fgo_toivanen2d.argtypes = [p, p, p, i, i, i, i, f, f, f, i]
+    lib.fgo_toivanen3d.argtypes = [p, p, p, i, i, i, i, i, p, f, f, f, i]
+    lib.fgo_toivanen2d.restype = ctypes.c_int
+    lib.fgo_toivanen3d.restype = ctypes.c_int
     lib.fgo_num_threads.restype = ctypes.c_int
     lib.fgo_set_num_threads.argtypes = [i]
     lib.fgo_set_num_threads.restype = None
@@ -180,6 +184,26 @@ def gsf3d(image, mask, theta, spacing, v, lamb, iterations):
     s = _spacing(spacing)
     _check(_load().fgo_gsf3d(_ptr(image), _ptr(mask), _ptr(out), C, mask.shape[1], D, H,
                              W, theta, _ptr(s), v, lamb, int(iterations)))
+    return out
+
+
+# ---- Toivanen's raster scan: accuracy cross-check ---------------------------
+# (geodis_toivanen.cpp; called like FastGeodisCpp.generalised_geodesic*_toivanen)
+
+def toivanen2d(image, mask, v, l_grad, l_eucl, iterations):
+    image, mask, out = _prep(image, mask, 2)
+    _, C, H, W = image.shape
+    _check(_load().fgo_toivanen2d(_ptr(image), _ptr(mask), _ptr(out), C, mask.shape[1], H, W,
+                                  v, l_grad, l_eucl, int(iterations)))
+    return out
+
+
+def toivanen3d(image, mask, spacing, v, l_grad, l_eucl, iterations):
+    image, mask, out = _prep(image, mask, 3)
+    _, C, D, H, W = image.shape
+    s = _spacing(spacing)
+    _check(_load().fgo_toivanen3d(_ptr(image), _ptr(mask), _ptr(out), C, mask.shape[1], D, H,
+                                  W, _ptr(s), v, l_grad, l_eucl, int(iterations)))
     return out
 
 

@@ -15,6 +15,7 @@
  *   - l1distance |a-b|, channel sum from 0.0f            fastgeodis_cpu.cpp:38-41, 88-105
  *   - signed = D(M) - D(1-M)                             fastgeodis.cpp:241-258
  *   - GSF: three signed transforms, thresholds +-theta   fastgeodis.cpp:319-343
+ *   - Toivanen's raster scan (accuracy cross-check)      geodis_toivanen.cpp:40-348
  *
  * The reference transposes tensors so that every sweep runs along the
  * outermost spatial axis; transposes only move data, so this restatement
@@ -389,5 +390,100 @@ int fgo_geodesic2d_layout(const float *img, const float *mask, float *out,
   }
   memcpy(out, da, n * sizeof(float));
   free(ia); free(ib); free(da); free(db);
+  return 0;
+}
+
+/* ---- Toivanen's raster scan (accuracy cross-check) -----------------------
+ * The reference's alternative CPU transform (geodis_toivanen.cpp:40-147 for
+ * 2D, :168-329 for 3D, drivers :149-166 and :331-348): one forward raster
+ * scan over the whole grid in memory order, each element relaxed from its
+ * causal neighbours (4 in 2D, 13 in 3D) with values already updated in the
+ * same scan, then one backward scan with the mirrored neighbour set;
+ * `iters` times.  Candidate, channel sum and min rule are those of the
+ * raster-scan transform; step lengths are sqrt of the squared spacings of
+ * the axes a step moves along, summed in (D, H, W) order from 0 (2D: 1 and
+ * sqrtf(2)).  Only mask channel 0 is transformed; out holds fl(v * mask).
+ * The neighbour sets of the two scans together are the full 8 / 26
+ * neighbourhood, the graph of the raster-scan transform, so both converge to
+ * the same values (bit for bit where the step tables agree). */
+
+/* 3D neighbour offsets (dz, dh, dw) of the forward and backward scans, in
+ * the reference's visiting order (geodis_toivanen.cpp:185-187, 212-214). */
+static const int tv_fwd3[13][3] = {
+    {-1, -1, -1}, {-1, -1, 0}, {-1, -1, 1}, {-1, 0, -1}, {-1, 0, 0}, {0, -1, -1}, {0, -1, 0},
+    {0, -1, 1},   {0, 0, -1},  {1, -1, -1}, {1, -1, 0},  {1, -1, 1}, {1, 0, -1}};
+static const int tv_bwd3[13][3] = {
+    {-1, 0, 1}, {-1, 1, -1}, {-1, 1, 0}, {-1, 1, 1}, {0, 0, 1}, {0, 1, -1}, {0, 1, 0},
+    {0, 1, 1},  {1, 0, 0},   {1, 0, 1},  {1, 1, -1}, {1, 1, 0}, {1, 1, 1}};
+/* 2D (dh, dw): geodis_toivanen.cpp:55-56, 102-103 */
+static const int tv_fwd2[4][3] = {{0, -1, -1}, {0, -1, 0}, {0, -1, 1}, {0, 0, -1}};
+static const int tv_bwd2[4][3] = {{0, 0, 1}, {0, 1, -1}, {0, 1, 0}, {0, 1, 1}};
+
+static void tv_scan(const float *img, float *d, int C, ptrdiff_t cs, int D, int H, int W,
+                    const int (*off)[3], const float *ld, int n, int forward,
+                    float l_grad, float l_eucl) {
+  const ptrdiff_t total = (ptrdiff_t)D * H * W;
+  for (ptrdiff_t t = 0; t < total; ++t) {
+    const ptrdiff_t p = forward ? t : total - 1 - t;
+    const int z = (int)(p / ((ptrdiff_t)H * W));
+    const int h = (int)((p / W) % H);
+    const int w = (int)(p % W);
+    float nd = d[p];
+    for (int k = 0; k < n; ++k) {
+      const int zq = z + off[k][0], hq = h + off[k][1], wq = w + off[k][2];
+      if (zq < 0 || zq >= D || hq < 0 || hq >= H || wq < 0 || wq >= W) continue;
+      const ptrdiff_t q = ((ptrdiff_t)zq * H + hq) * W + wq;
+      float l;
+      if (C == 1) {
+        l = fabsf(img[p] - img[q]);
+      } else {
+        l = 0.0f;
+        for (int c = 0; c < C; ++c) l += fabsf(img[c * cs + p] - img[c * cs + q]);
+      }
+      const float cur = (d[q] + l_eucl * ld[k]) + l_grad * l;
+      nd = ref_min(nd, cur);
+    }
+    d[p] = nd;
+  }
+}
+
+/* sqrt of the squared spacings of the axes an offset moves along, summed in
+ * (D, H, W) order (geodis_toivanen.cpp:189-209, 216-236). */
+static void tv_table(const int (*off)[3], int n, const float sq[3], float *ld) {
+  for (int k = 0; k < n; ++k) {
+    float s = 0.0f;
+    for (int a = 0; a < 3; ++a)
+      if (off[k][a] != 0) s += sq[a];
+    ld[k] = sqrtf(s);
+  }
+}
+
+int fgo_toivanen2d(const float *img, const float *mask, float *out, int C, int Cm, int H,
+                   int W, float v, float l_grad, float l_eucl, int iters) {
+  if (C < 1 || Cm < 1 || H < 1 || W < 1) return -1;
+  const float r2 = sqrtf(2.0f);
+  const float ldf[4] = {r2, 1.0f, r2, 1.0f}, ldb[4] = {1.0f, r2, 1.0f, r2};
+  init_dist(mask, out, (size_t)Cm * H * W, v);
+  for (int it = 0; it < iters; ++it) {
+    tv_scan(img, out, C, (ptrdiff_t)H * W, 1, H, W, tv_fwd2, ldf, 4, 1, l_grad, l_eucl);
+    tv_scan(img, out, C, (ptrdiff_t)H * W, 1, H, W, tv_bwd2, ldb, 4, 0, l_grad, l_eucl);
+  }
+  return 0;
+}
+
+int fgo_toivanen3d(const float *img, const float *mask, float *out, int C, int Cm, int D,
+                   int H, int W, const float spacing[3], float v, float l_grad, float l_eucl,
+                   int iters) {
+  if (C < 1 || Cm < 1 || D < 1 || H < 1 || W < 1) return -1;
+  float sq[3], ldf[13], ldb[13];
+  for (int k = 0; k < 3; ++k) sq[k] = spacing[k] * spacing[k];
+  tv_table(tv_fwd3, 13, sq, ldf);
+  tv_table(tv_bwd3, 13, sq, ldb);
+  init_dist(mask, out, (size_t)Cm * D * H * W, v);
+  const ptrdiff_t cs = (ptrdiff_t)D * H * W;
+  for (int it = 0; it < iters; ++it) {
+    tv_scan(img, out, C, cs, D, H, W, tv_fwd3, ldf, 13, 1, l_grad, l_eucl);
+    tv_scan(img, out, C, cs, D, H, W, tv_bwd3, ldb, 13, 0, l_grad, l_eucl);
+  }
   return 0;
 }
