@@ -40,7 +40,7 @@ DEFAULT_CANDIDATES_3D: Tuple[Tuple[int, int, int], ...] = (
     (1, 3, 2), (1, 6, 2), (1, 10, 4), (1, 8, 1),
 )
 DEFAULT_CANDIDATES_2D: Tuple[Tuple[int, int, int], ...] = (
-    (1, 0, 0), (1, 8, 0), (1, 16, 0), (1, 24, 0),
+    (1, 0, 0), (1, 6, 0), (1, 12, 0), (1, 24, 0), (1, 30, 0),
 )
 
 _cache: Dict[str, dict] = {}

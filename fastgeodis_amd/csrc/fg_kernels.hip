@@ -1873,8 +1873,8 @@ struct Runner {
   // or GSF call) are planned together: the deepest halo at which all their
   // tiles still fit the device, and the shallowest one when none does, as a
   // deeper halo then only adds window arithmetic (a 32 x 64 window computes
-  // 1.7 x its own tile at K = 5 and 3.9 x at K = 10; a 64-lane row 1.3 x at
-  // K = 8 and 4 x at K = 24).  3D tiles fill a CU each; 2D rows are one-wave
+  // 1.7 x its own tile at K = 5 and 3.9 x at K = 10; a 64-lane row 1.23 x at
+  // K = 6 and 16 x at K = 30).  3D tiles fill a CU each; 2D rows are one-wave
   // tiles bound by latency, so they fit one per SIMD.
   std::vector<int> pass_halos(const std::vector<Xform *> &active, int pass) {
     std::vector<int> ks(active.size(), 0);

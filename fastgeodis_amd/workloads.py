@@ -139,6 +139,7 @@ def run_c5(rates=(50, 200, 500, 1000, 2000), requests: int = 1000, sessions: int
     try:
         serving.poisson_load(srv, 20.0, 20)  # warm-up
         for q in rates:
+            print(f"[workloads] c5: {requests} requests at {q:g} clicks/s", file=sys.stderr, flush=True)
             lat, wall, bs = serving.poisson_load(srv, float(q), requests, seed=int(q))
             lat = np.sort(np.asarray(lat)) * 1e3
             rows.append({"qps_offered": q, "requests": len(lat), "throughput_rps": len(lat) / wall,
