@@ -71,6 +71,9 @@ constexpr int kMaxK = 30;
 constexpr int kHaloPollDelay = 24;
 // The same for one-wave row tiles (two neighbours each): 512 clocks.
 constexpr int kRowPollDelay = 8;
+// Multi-wave tiles skip the slab loads of window rows whose values at that
+// step are not exact (see load_rows).
+constexpr bool kSkipEdgeLoads = true;
 // Waves that perform the halo exchanges in kernels with exchange waves.
 constexpr int kCommWaves = 4;
 // Buckets of the exchange-wait histogram: bucket b counts waits of
@@ -558,26 +561,50 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
       return m;
     }
   };
-  auto load_rows = [&](Slab<R, CT> &S, const Cursor &cur, int r0, int r1) {
+  // Slab offsets per ring slot.  With K == P a block is one turn of the ring
+  // and slot t always holds block step t + 1.  At block step j the window
+  // rows less than j rows from a window edge that faces a neighbour tile
+  // depend on positions outside the window: their values are not exact and
+  // are never stored or published (the next exchange replaces them), so
+  // their loads get an out-of-range offset (the load returns 0 without a
+  // memory access).  The image rows of a block's last step (slot P - 1) are
+  // all loaded: after the exchange they are the front image of the next
+  // block's first step, whose rows are all exact again.  Computed once, so
+  // the steps pay nothing for it; with K > P every row is loaded.
+  constexpr int SK = (kSkipEdgeLoads && NW > 1) ? P : 1;
+  unsigned slot_ofs[SK][R];
+#pragma unroll
+  for (int t = 0; t < SK; ++t)
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+      const int wr = w * R + r;  // window row (uniform)
+      const bool skip = SK > 1 && K == P && nTB > 1 && min(wr, NWR - 1 - wr) <= t;
+      slot_ofs[t][r] = skip ? 0xFFFFFFFFu : rofs[r + 1];
+    }
+  // t: the ring slot the slab fills, or -1 (every row loaded)
+  auto load_rows = [&](Slab<R, CT> &S, const Cursor &cur, int r0, int r1, int t) {
     const __amdgpu_buffer_rsrc_t rd = cursor_rsrc(INIT ? Dd.init_src : Dd.d, cur);
     __amdgpu_buffer_rsrc_t ri[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) ri[c] = cursor_rsrc(Dd.img + c * cs, cur);
 #pragma unroll
     for (int r = 0; r < R + 2; ++r) {
+      const bool slotted = SK > 1 && t >= 0 && r >= 1 && r <= R;
+      const unsigned od = slotted ? slot_ofs[t][r - 1] : at(rofs[r], cur);
+      const unsigned oi = (slotted && t < P - 1) ? slot_ofs[t][r - 1] : at(rofs[r], cur);
       // the one-row kernel (a plane of one row) uses only its own image row
       if (r >= r0 && r < r1 && (!ROW1 || r == 1)) {
 #pragma unroll
         for (int c = 0; c < CT; ++c)
-          S.I[c][r] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ri[c], (int)at(rofs[r], cur), 0, 0));
+          S.I[c][r] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(ri[c], (int)oi, 0, 0));
       }
       if (r >= 1 && r <= R)
-        S.d[r - 1] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rd, (int)at(rofs[r], cur), 0, 0));
+        S.d[r - 1] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(rd, (int)od, 0, 0));
     }
   };
-  auto load_slab = [&](Slab<R, CT> &S, const Cursor &cur) {
-    if constexpr (NW > 1) load_rows(S, cur, 1, R + 1);
-    else load_rows(S, cur, 0, R + 2);
+  auto load_slab = [&](Slab<R, CT> &S, const Cursor &cur, int t) {
+    if constexpr (NW > 1) load_rows(S, cur, 1, R + 1, t);
+    else load_rows(S, cur, 0, R + 2, t);
   };
 
   // per-row validity of this lane (pad positions hold +inf in the front) and
@@ -595,7 +622,7 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   // S[P - 1].  Slabs stay in fixed registers this way.
   float Fd[R];  // front distance rows 0..R-1
   Slab<R, CT> S[P];
-  load_rows(S[P - 1], cursor_of(-1), 0, R + 2);  // front slice: not modified by this sweep
+  load_rows(S[P - 1], cursor_of(-1), 0, R + 2, -1);  // front slice: not modified by this sweep
 #pragma unroll
   for (int r = 0; r < R; ++r) Fd[r] = pvalid[r] ? init_of(S[P - 1].d[r]) : kInf;
   if constexpr (INIT) {
@@ -613,7 +640,7 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     for (int r = 0; r < R; ++r) xtile[P - 1][w * R + r][lane] = Fd[r];
   }
 #pragma unroll
-  for (int p = 0; p < P - 1; ++p) load_slab(S[p], cursor_of(p));
+  for (int p = 0; p < P - 1; ++p) load_slab(S[p], cursor_of(p), p);
   // the slices that the loop stores (step s) and prefetches (step s + P - 1)
   Cursor cur_st = cursor_of(0), cur_pf = cursor_of(P - 1);
 
@@ -633,7 +660,7 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     (void)blk;
 #pragma unroll
     for (int r = 0; r < R; ++r) xfront[w * R + r][lane] = Fd[r];
-    load_slab(pf_slab, cur_pf);
+    load_slab(pf_slab, cur_pf, P - 2);
     advance(cur_pf);
     __syncthreads();  // front visible to the exchange wave
     __syncthreads();  // halo values written by it
@@ -691,7 +718,7 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
       x[r] = 0;
       if (need[r]) x[r] = __hip_atomic_load(src[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    load_slab(pf_slab, cur_pf);
+    load_slab(pf_slab, cur_pf, P - 2);
     advance(cur_pf);
     // after an expiry this wave waits no more: the sweep then finishes
     // (with wrong values, reported through the status word) instead of
@@ -722,8 +749,9 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     }
   };
 
-  // One slice: Sl holds the new slice, Fr the previous one (front image).
-  // Afterwards Fr is refilled with the slice of step s + P - 1.
+  // One slice: Sl holds the new slice (ring slot `slot`), Fr the previous
+  // one (front image).  Afterwards Fr is refilled with the slice of step
+  // s + P - 1.
   auto step = [&](Slab<R, CT> &Sl, Slab<R, CT> &Fr, int s, int slot, bool prefetch) {
     // steps past the end of the sweep (the last block is padded to K steps)
     // compute on slices of zeros and store nothing
@@ -802,7 +830,7 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     // prefetch without a branch: past the end of the sweep the loads return 0
     // and the data is never consumed (only steps that store nothing use it)
     if (prefetch) {
-      load_slab(Fr, cur_pf);
+      load_slab(Fr, cur_pf, (slot + P - 1) % P);
       advance(cur_pf);
     }
   };
