@@ -16,8 +16,10 @@
 
 #include <cstdlib>
 #include <iostream>
+#include <map>
 #include <stdexcept>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "fastgeodis_hip.h"
@@ -377,6 +379,113 @@ std::tuple<std::vector<int64_t>, int64_t> profile_exchange_histogram() {
   return {std::vector<int64_t>(h, h + 32), (int64_t)khz};
 }
 
+// Continuous batching at pass granularity (fg_stepper_*).  The stepper owns
+// its exchange and descriptor buffers; every added instance keeps its
+// tensors (image, mask, output, workspace, transposed image) alive until a
+// step reports it done.  Steps run on the device's current stream.
+class Stepper {
+ public:
+  Stepper(int64_t halo_bytes, int64_t max_transforms, int64_t device) : dev_(device) {
+    c10::DeviceGuard guard(c10::Device(c10::kCUDA, (c10::DeviceIndex)device));
+    const auto opt = torch::TensorOptions().dtype(torch::kUInt8).device(torch::kCUDA, (c10::DeviceIndex)device);
+    const size_t bb = fg_stepper_blob_bytes((int)max_transforms);
+    halo_ = torch::empty({halo_bytes}, opt);
+    blob_ = torch::empty({(int64_t)bb}, opt);
+    raise_for(fg_stepper_create(&s_, halo_.data_ptr(), (size_t)halo_bytes, blob_.data_ptr(), bb));
+  }
+  ~Stepper() {
+    if (s_) fg_stepper_destroy(s_);
+  }
+  Stepper(const Stepper &) = delete;
+  Stepper &operator=(const Stepper &) = delete;
+
+  // (ticket, output); ticket -1: the exchange buffer cannot take the instance
+  // now (retry after a step has retired others)
+  std::tuple<int64_t, torch::Tensor> add(const torch::Tensor &image_in, const torch::Tensor &mask_in,
+                                         const std::vector<float> &spacing, int64_t op, double v, double lg,
+                                         double le, int64_t iterations,
+                                         const c10::optional<torch::Tensor> &image_t_opt) {
+    const int nd = (int)image_in.dim() - 2;
+    if (nd != 2 && nd != 3) throw std::invalid_argument("need a 2D or 3D instance");
+    check_input_dimensions(image_in, mask_in, nd + 2);
+    check_cuda(image_in);
+    check_cuda(mask_in);
+    check_float(image_in, "image");
+    check_float(mask_in, "mask");
+    if (image_in.device().index() != dev_ || mask_in.device().index() != dev_)
+      throw std::invalid_argument("instance tensors must be on the stepper's device");
+    if (nd == 3 && spacing.size() != 3)
+      throw std::invalid_argument("function only supports 3D spacing inputs, received " + std::to_string(spacing.size()));
+    c10::DeviceGuard guard(image_in.device());
+    Held h;
+    h.image = image_in.contiguous();
+    h.mask = mask_in.contiguous();
+    h.out = torch::empty_like(h.mask, torch::MemoryFormat::Contiguous);
+    fg_instance f{};
+    f.image = h.image.data_ptr<float>();
+    f.mask = h.mask.data_ptr<float>();
+    f.out = h.out.data_ptr<float>();
+    f.C = (int)h.image.size(1);
+    f.Cm = (int)h.mask.size(1);
+    f.is2d = nd == 2;
+    f.D = nd == 2 ? 1 : (int)h.image.size(2);
+    f.H = (int)h.image.size(nd == 2 ? 2 : 3);
+    f.W = (int)h.image.size(nd == 2 ? 3 : 4);
+    f.op = (int)op;
+    f.iterations = (int)iterations;
+    for (int k = 0; k < 3; ++k) f.spacing[k] = nd == 3 ? spacing[k] : 1.0f;
+    f.v = (float)v;
+    f.l_grad = (float)lg;
+    f.l_eucl = (float)le;
+    if (image_t_opt.has_value() && image_t_opt->defined() && image_t_opt->numel() > 0) {
+      const torch::Tensor &image_t = *image_t_opt;
+      check_float(image_t, "transposed image");
+      std::vector<int64_t> want{1, h.image.size(1)};
+      for (int k = nd + 1; k >= 2; --k) want.push_back(h.image.size(k));
+      if (image_t.device() != h.image.device() || !image_t.is_contiguous() || image_t.sizes().vec() != want)
+        throw std::invalid_argument("transposed image must be a contiguous (1, C, W, H[, D]) tensor on the image's device");
+      h.image_t = image_t;
+      f.imageT = image_t.data_ptr<float>();
+    }
+    size_t halo_share = 0;
+    const size_t wsb = fg_stepper_op_bytes(&f, &halo_share);
+    if (wsb == 0) raise_for(FG_EINVAL);
+    h.ws = torch::empty({(int64_t)wsb}, h.image.options().dtype(torch::kUInt8));
+    int ticket = 0;
+    const int rc = fg_stepper_add(s_, &f, h.ws.data_ptr(), wsb, &ticket);
+    if (rc == FG_EWORKSPACE && halo_share > 0) return {-1, torch::Tensor()};
+    raise_for(rc);
+    held_[ticket] = h;
+    return {ticket, h.out};
+  }
+
+  // one step on the current stream; the tickets of the instances it completed
+  std::vector<int64_t> step() {
+    c10::DeviceGuard guard(c10::Device(c10::kCUDA, (c10::DeviceIndex)dev_));
+    std::vector<int> done(held_.size() + 1);
+    int n = 0;
+    void *stream = (void *)c10::hip::getCurrentHIPStream((c10::DeviceIndex)dev_).stream();
+    raise_for(fg_stepper_step(s_, stream, done.data(), (int)done.size(), &n));
+    std::vector<int64_t> out;
+    for (int i = 0; i < n; ++i) {
+      out.push_back(done[i]);
+      held_.erase(done[i]);  // later work on this stream is ordered after the step
+    }
+    return out;
+  }
+
+  int64_t active() const { return fg_stepper_active(s_); }
+
+ private:
+  struct Held {
+    torch::Tensor image, mask, out, ws, image_t;
+  };
+  int64_t dev_;
+  torch::Tensor halo_, blob_;
+  fg_stepper *s_ = nullptr;
+  std::map<int, Held> held_;
+};
+
 int64_t workspace_bytes(int64_t op, int64_t C, int64_t D, int64_t H, int64_t W) {
   return (int64_t)fg_workspace_bytes((int)op, (int)C, (int)D, (int)H, (int)W);
 }
@@ -409,4 +518,13 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m) {
   m.def("set_wait_limit", &set_wait_limit, "Spin limit of the bounded halo waits for this host thread (0 = default)");
   m.def("release", []() { fg_release(); }, "Free the library's pinned memory, events and graphs (process exit)");
   m.def("set_graphs", &set_graphs, "Graph replay of repeated launch sequences on this host thread (default on)");
+  pybind11::class_<Stepper>(m, "Stepper", "Continuous batching at pass granularity (unsigned and signed transforms)")
+      .def(pybind11::init<int64_t, int64_t, int64_t>(), pybind11::arg("halo_bytes"), pybind11::arg("max_transforms"),
+           pybind11::arg("device"))
+      .def("add", &Stepper::add, "Add an instance: (ticket, output); ticket -1 when the exchange buffer is full",
+           pybind11::arg("image"), pybind11::arg("mask"), pybind11::arg("spacing"), pybind11::arg("op"),
+           pybind11::arg("v"), pybind11::arg("lg"), pybind11::arg("le"), pybind11::arg("iterations"),
+           pybind11::arg("image_t") = pybind11::none())
+      .def("step", &Stepper::step, "Enqueue one step on the current stream; tickets it completed")
+      .def("active", &Stepper::active, "Instances in flight");
 }

@@ -1223,6 +1223,7 @@ struct Xform {
   float e1[3][9];  // per pass: D, H, W (2D: [1] rows pass, [2] columns pass)
   bool lamb1, fast;
   int iters;
+  int it = 0;  // the iteration being recorded
   bool xst;  // this iteration's H-backward sweep writes dT directly (no transpose before the W pass)
   // The first sweep initialises the distance from the mask (init_mode 1: v*m,
   // 2: v*(1-m)) instead of reading it; set only when that sweep is tiled.
@@ -1572,7 +1573,9 @@ int status_error(int st, const char *when) {
                                   ": a tile's bounded halo wait expired (another persistent kernel may have held "
                                   "compute units); the results of that call are invalid");
   if (st == FG_EINVAL)
-    return fail(FG_EHIP, std::string(when) + ": a sweep launch was refused (halo depth not a multiple of its ring)");
+    return fail(FG_EHIP, std::string(when) +
+                             ": a sweep launch was refused (halo depth not a multiple of its ring, or a one-row "
+                             "kernel on a plane of several rows)");
   return fail(FG_EHIP, std::string(when) + ": device status " + std::to_string(st));
 }
 
@@ -1693,11 +1696,11 @@ struct Runner {
   unsigned tag_base = 0;
   Recorder rec;
 
-  int cur_it = 0;  // iteration being recorded
+  bool no_graph = false;  // replay the launches directly (sequences that do not repeat)
 
   void fill_desc(const Xform &x, int pass, int dir, SweepDesc &s, bool xst = false) {
     std::memset(&s, 0, sizeof s);
-    if (x.init_mode && cur_it == 0 && pass == first_pass(x) && dir > 0) {
+    if (x.init_mode && x.it == 0 && pass == first_pass(x) && dir > 0) {
       s.init_src = x.init_src;
       s.init_v = x.init_v;
       s.init_mode = x.init_mode;
@@ -1964,7 +1967,7 @@ struct Runner {
         int nA, nB, nE;
         pass_plane(*x, pass, nA, nB, nE);
         if (nA <= 1) continue;
-        const bool in = x->init_mode && cur_it == 0 && pass == first_pass(*x) && dir > 0;
+        const bool in = x->init_mode && x->it == 0 && pass == first_pass(*x) && dir > 0;
         if (ks[i] == 0) {
           if (in) return fail(FG_EINVAL, "internal: an initialising sweep was planned on the per-slice engine");
           record_per_slice(*x, pass, dir);
@@ -2032,45 +2035,54 @@ struct Runner {
     return FG_OK;
   }
 
+  // The transposes around a W pass: into its layout (unless the H pass's
+  // backward sweep wrote it) and back, the last iteration's carrying the
+  // group's epilogue (members of a group contribute through its first one).
+  void w_pass_transposes(const std::vector<Xform *> &active, std::vector<XposeDesc> &to,
+                         std::vector<XposeDesc> &back) {
+    for (auto *x : active) {
+      int D, H, W;
+      native_dims(*x, D, H, W);
+      if (!x->xst) to.push_back(xd(x->d, x->dT, 1, D, H, W));
+      const bool last = x->it == x->iters - 1;
+      if (last && x->fin_member) continue;  // part of its group's combined transpose
+      XposeDesc b = xd(x->dT, x->d, 1, W, H, D);
+      if (last && x->fin_kind != XK_COPY) {
+        b.kind = x->fin_kind;
+        b.src2 = x->fin_src[0];
+        b.src3 = x->fin_src[1];
+        b.src4 = x->fin_src[2];
+        b.dst = x->fin_dst[0];
+        b.dst2 = x->fin_dst[1];
+        b.dst3 = x->fin_dst[2];
+        b.dst4 = x->fin_dst[3];
+        b.v = x->fin_v;
+        b.theta = x->fin_theta;
+      }
+      back.push_back(b);
+    }
+  }
+
   int record_run(std::vector<Xform> &xs) {
     int max_it = 0;
     for (auto &x : xs) max_it = std::max(max_it, x.iters);
     for (int it = 0; it < max_it; ++it) {
-      cur_it = it;
       std::vector<Xform *> active;
       for (auto &x : xs)
-        if (x.iters > it) active.push_back(&x);
+        if (x.iters > it) {
+          x.it = it;
+          active.push_back(&x);
+        }
       int rc;
       for (auto *x : active) x->xst = xstore_candidate(*x);
       if ((rc = sweep_pass(active, 0))) return rc;
       if ((rc = sweep_pass(active, 1))) return rc;
       std::vector<XposeDesc> to, back;
-      for (auto *x : active) {
-        int D, H, W;
-        native_dims(*x, D, H, W);
-        if (!x->xst) to.push_back(xd(x->d, x->dT, 1, D, H, W));
-        const bool last = it == x->iters - 1;
-        if (last && x->fin_member) continue;  // part of its group's combined transpose
-        XposeDesc b = xd(x->dT, x->d, 1, W, H, D);
-        if (last && x->fin_kind != XK_COPY) {
-          b.kind = x->fin_kind;
-          b.src2 = x->fin_src[0];
-          b.src3 = x->fin_src[1];
-          b.src4 = x->fin_src[2];
-          b.dst = x->fin_dst[0];
-          b.dst2 = x->fin_dst[1];
-          b.dst3 = x->fin_dst[2];
-          b.dst4 = x->fin_dst[3];
-          b.v = x->fin_v;
-          b.theta = x->fin_theta;
-        }
-        back.push_back(b);
-      }
+      w_pass_transposes(active, to, back);
       record_xpose(to);
       if ((rc = sweep_pass(active, 2))) return rc;
       record_xpose(back);
     }
-    cur_it = 0;
     return FG_OK;
   }
 
@@ -2181,7 +2193,7 @@ struct Runner {
 
     // graph replay of a repeated sequence (not while profiling per launch, not
     // for the per-slice engine's hundreds of launches, never without allocating)
-    if (g_graph_on && !g_prof_on && !g_prof_xhist && !slice && !no_alloc && n >= 2) {
+    if (g_graph_on && !no_graph && !g_prof_on && !g_prof_xhist && !slice && !no_alloc && n >= 2) {
       uint64_t key = 1469598103934665603ull;
       auto mix = [&](uint64_t x) { key = (key ^ x) * 1099511628211ull; };
       for (size_t i = 0; i < n; ++i) {
@@ -2629,6 +2641,193 @@ int run_single(OpSpec o, void *ws, size_t wsb, void *stream) {
   return run_ops(ops, ws, wsb, (hipStream_t)stream);
 }
 
+// ---------------------------------------------------------------------------
+// Stepper: transforms that advance one pass pair per step, so that new ones
+// join between steps (continuous batching of a serving loop).  Each step
+// records, for every active transform, its next pass pair (forward and
+// backward sweep) in launches shared with every other transform at that
+// pass, plus the transposes around W passes; a transform that has run its
+// last W pass leaves with its epilogue applied.  Results equal the same
+// transform run alone: a transform computes exactly the launches of its
+// plain call, only in the company of others.
+// ---------------------------------------------------------------------------
+
+struct StepOp {
+  OpSpec o;
+  int ticket = 0;
+  bool started = false;   // its first step (image transpose, mask channels >= 1) recorded
+  int it = 0;             // iteration of its next pass pair
+  int pass = 0;           // next pass: 0 D, 1 H, 2 W (2D: 1 rows, 2 columns)
+  float *imgT = nullptr;  // the image in the W pass layout
+  std::vector<Xform> xs;  // 1 (unsigned) or 2 (signed: D(M), D(1-M))
+  size_t halo_entries = 0;
+};
+
+struct Stepper {
+  int device = -1;
+  unsigned long long *halo = nullptr;
+  size_t halo_cap = 0;  // entries
+  unsigned char *blob = nullptr;
+  size_t blob_cap = 0;
+  bool halo_clear = false;  // the first step clears the exchange buffer
+  unsigned tag_base = 0;
+  int next_ticket = 1;
+  size_t halo_used = 0;  // entries reserved by the active transforms
+  std::vector<StepOp> ops;
+};
+
+size_t stepper_blob_bytes(int max_transforms) {
+  const size_t n = (size_t)std::max(max_transforms, 1);
+  // per step: 3 passes x 2 directions of sweeps, 3 transpose launches, 2 elementwise
+  return 6 * (n * sizeof(SweepDesc) + 256) * 2 + 3 * (n * sizeof(XposeDesc) + 256) +
+         2 * (n * sizeof(EwDesc) + 256) + 4096;
+}
+
+int stepper_runner(const Stepper &st, Runner &run) {
+  int rc = query_device(run.dev);
+  if (rc) return rc;
+  if (run.dev.id != st.device) return fail(FG_EINVAL, "the stepper belongs to another device");
+  run.K = current_halo();
+  run.engine = g_tune.engine;
+  run.max_groups = g_tune.max_groups;
+  run.halo_cap = st.halo_cap;
+  run.tag_base = st.tag_base;
+  run.no_graph = true;
+  return FG_OK;
+}
+
+int stepper_add(Stepper &st, const OpSpec &o, void *ws, size_t wsb, int *ticket) {
+  int rc = validate(o);
+  if (rc) return rc;
+  if (o.op != FG_OP_UNSIGNED && o.op != FG_OP_SIGNED) return fail(FG_EINVAL, "the stepper runs unsigned and signed transforms");
+  if (o.iters < 1) return fail(FG_EINVAL, "the stepper needs iterations >= 1");
+  if (!ws || wsb < op_buffers_bytes(o)) return fail(FG_EWORKSPACE, "transform workspace too small");
+  Runner run{};
+  if ((rc = stepper_runner(st, run))) return rc;
+  const size_t hneed = halo_entries_for({o}, run.K);
+  if (st.halo_used + hneed > st.halo_cap) return fail(FG_EWORKSPACE, "the stepper's exchange buffer is full");
+
+  StepOp op;
+  op.o = o;
+  const long long N = vox(o);
+  const int nd = n_dist(o.op);
+  Carver cv(ws, wsb, false);
+  op.imgT = o.imageT ? const_cast<float *>(o.imageT) : (float *)cv.take((size_t)o.C * N * sizeof(float));
+  float *dist[2] = {o.out, nd > 1 ? (float *)cv.take((size_t)N * sizeof(float)) : nullptr};
+  float *dT[2] = {nullptr, nullptr};
+  for (int k = 0; k < nd; ++k) dT[k] = (float *)cv.take((size_t)N * sizeof(float));
+  if (!cv.fits()) return fail(FG_EWORKSPACE, "transform workspace too small");
+  for (int k = 0; k < nd; ++k) {
+    Xform x{};
+    make_xform(o, o.lg, o.le, x);
+    x.img = o.image;
+    x.imgT = op.imgT;
+    x.d = dist[k];
+    x.dT = dT[k];
+    op.xs.push_back(x);
+  }
+  // the initialisation is computed by the first sweep, which therefore must
+  // run on the tiled engine
+  bool tiled = false;
+  if ((rc = run.first_sweep_tiled(op.xs[0], tiled))) return rc;
+  if (!tiled) return fail(FG_EINVAL, "the stepper needs a first sweep on the tiled engine (plane too large)");
+  for (int k = 0; k < nd; ++k) {
+    op.xs[k].init_mode = (k == 0) ? 1 : 2;
+    op.xs[k].init_src = o.mask;
+    op.xs[k].init_v = o.v;
+  }
+  if (o.op == FG_OP_SIGNED) {
+    op.xs[0].fin_kind = XK_SIGNED;
+    op.xs[0].fin_src[0] = op.xs[1].dT;
+    op.xs[0].fin_dst[0] = o.out;
+    op.xs[0].fin_v = o.v;
+    op.xs[1].fin_member = true;
+  }
+  op.pass = first_pass(op.xs[0]);
+  op.ticket = st.next_ticket++;
+  if (st.next_ticket <= 0) st.next_ticket = 1;
+  op.halo_entries = hneed;
+  st.halo_used += hneed;
+  *ticket = op.ticket;
+  st.ops.push_back(std::move(op));
+  return FG_OK;
+}
+
+int stepper_step(Stepper &st, hipStream_t stream, int *done, int cap, int *ndone) {
+  *ndone = 0;
+  if (st.ops.empty()) return FG_OK;
+  Runner run{};
+  int rc = stepper_runner(st, run);
+  if (rc) return rc;
+  DeviceState &dstate = device_state(run.dev.id);
+  if ((rc = ensure_status_word(dstate))) return rc;
+  if ((rc = status_error(take_status(dstate), "an earlier call on this device"))) return rc;
+  if (!st.halo_clear) run.record_zero_halo();
+
+  // first steps: image transposes and mask channels >= 1 (closed form)
+  std::vector<XposeDesc> ximg;
+  std::vector<EwDesc> chans;
+  for (auto &op : st.ops) {
+    if (op.started) continue;
+    const OpSpec &o = op.o;
+    if (!o.imageT) {
+      if (o.is2d) ximg.push_back(Runner::xd(o.image, op.imgT, o.C, o.H, 1, o.W));
+      else ximg.push_back(Runner::xd(o.image, op.imgT, o.C, o.D, o.H, o.W));
+    }
+    if (o.Cm > 1) {
+      EwDesc c{};
+      c.n0 = vox(o);
+      c.Cm = o.Cm;
+      c.op = o.op;
+      c.mask = o.mask;
+      c.out = o.out;
+      c.v = o.v;
+      c.theta = o.theta;
+      c.kind = EW_CHANNELS;
+      c.nelem = vox(o) * (o.Cm - 1);
+      chans.push_back(c);
+    }
+  }
+  run.record_xpose(ximg);
+  run.record_ew(chans);
+
+  std::vector<Xform *> act[3];
+  for (auto &op : st.ops)
+    for (auto &x : op.xs) {
+      x.it = op.it;
+      act[op.pass].push_back(&x);
+    }
+  for (auto *x : act[1]) x->xst = run.xstore_candidate(*x);
+  std::vector<XposeDesc> to, back;
+  run.w_pass_transposes(act[2], to, back);
+  run.record_xpose(to);
+  for (int pass = 0; pass < 3; ++pass)
+    if (!act[pass].empty() && (rc = run.sweep_pass(act[pass], pass))) return rc;
+  run.record_xpose(back);
+  if ((rc = run.replay(stream, st.blob, st.blob_cap, st.halo, dstate.status_dev, false))) return rc;
+  st.halo_clear = true;
+  st.tag_base = run.tag_base;
+
+  // advance; transforms past their last W pass leave
+  std::vector<StepOp> keep;
+  for (auto &op : st.ops) {
+    op.started = true;
+    if (op.pass == 2) {
+      op.pass = first_pass(op.xs[0]);
+      if (++op.it == op.o.iters) {
+        if (*ndone < cap) done[(*ndone)++] = op.ticket;
+        st.halo_used -= op.halo_entries;
+        continue;
+      }
+    } else {
+      ++op.pass;
+    }
+    keep.push_back(std::move(op));
+  }
+  st.ops.swap(keep);
+  return FG_OK;
+}
+
 
 }  // namespace
 
@@ -2678,7 +2877,7 @@ int noalloc_scratch(DeviceState &ds, unsigned char **out) {
 
 extern "C" {
 
-int fg_abi_version(void) { return 3; }
+int fg_abi_version(void) { return 4; }
 
 const char *fg_last_error(void) { return g_err.c_str(); }
 
@@ -2799,6 +2998,54 @@ int fg_geodesic_batch_f32(const fg_instance *inst, int n, void *workspace, size_
   if (ops.empty()) return FG_OK;
   return run_ops(ops, workspace, workspace_bytes, (hipStream_t)stream);
 }
+
+struct fg_stepper {
+  Stepper impl;
+};
+
+size_t fg_stepper_blob_bytes(int max_transforms) { return stepper_blob_bytes(max_transforms); }
+
+size_t fg_stepper_op_bytes(const fg_instance *inst, size_t *halo_bytes) {
+  std::vector<OpSpec> ops;
+  if (!inst || to_specs(inst, 1, ops)) return 0;
+  if (halo_bytes) *halo_bytes = halo_entries_for(ops, current_halo()) * sizeof(unsigned long long);
+  return op_buffers_bytes(ops[0]);
+}
+
+int fg_stepper_create(fg_stepper **out, void *halo, size_t halo_bytes, void *blob, size_t blob_bytes) {
+  if (!out || !halo || !blob) return fail(FG_EINVAL, "null argument");
+  if (halo_bytes < 16 || blob_bytes < 4096) return fail(FG_EWORKSPACE, "stepper buffers too small");
+  int dev = 0;
+  FG_HIP(hipGetDevice(&dev));
+  fg_stepper *p = new (std::nothrow) fg_stepper();
+  if (!p) return fail(FG_EHIP, "out of host memory");
+  p->impl.device = dev;
+  p->impl.halo = (unsigned long long *)halo;
+  p->impl.halo_cap = (halo_bytes - 8) / sizeof(unsigned long long);
+  p->impl.blob = (unsigned char *)blob;
+  p->impl.blob_cap = blob_bytes;
+  *out = p;
+  return FG_OK;
+}
+
+int fg_stepper_add(fg_stepper *s, const fg_instance *inst, void *workspace, size_t workspace_bytes,
+                   int *ticket) {
+  if (!s || !inst || !ticket) return fail(FG_EINVAL, "null argument");
+  std::vector<OpSpec> ops;
+  int rc = to_specs(inst, 1, ops);
+  if (rc) return rc;
+  return stepper_add(s->impl, ops[0], workspace, workspace_bytes, ticket);
+}
+
+int fg_stepper_step(fg_stepper *s, void *stream, int *done, int cap, int *ndone) {
+  if (!s || !ndone || (cap > 0 && !done)) return fail(FG_EINVAL, "null argument");
+  if (cap < (int)s->impl.ops.size()) return fail(FG_EINVAL, "done array smaller than the active transforms");
+  return stepper_step(s->impl, (hipStream_t)stream, done, cap, ndone);
+}
+
+int fg_stepper_active(const fg_stepper *s) { return s ? (int)s->impl.ops.size() : 0; }
+
+void fg_stepper_destroy(fg_stepper *s) { delete s; }
 
 int fg_set_voxels_f32(const fg_voxel_set *items, int n, void *stream) {
   if (n < 0 || (n > 0 && !items)) return fail(FG_EINVAL, "bad voxel array");

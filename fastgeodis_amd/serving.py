@@ -5,27 +5,30 @@ voxels to a session's soft mask (foreground -> 1, background -> 0; the mask
 starts at 0.5, which makes ``v * 0.5`` effectively infinite) and asks for
 ``signed_generalised_geodesic3d(image, mask, spacing, v, lamb, iter)``.
 
-Continuous batching: a scheduler thread keeps a queue of requests.  Whenever
-the device can take more work it admits queued requests, in arrival order,
-while their estimated device memory fits the HBM budget and at most one
-request per session is in the batch (a later click on the same session
-waits for the next batch, because it must see the earlier click's edit).
-Admitted requests run together through the batched entry point, so their
-sweeps share the same persistent launches.  Requests that arrive while a
-batch is on the GPU are admitted into the next batch as soon as the current
-one completes; two batches are kept in flight so the device never waits for
-the host.  Every request's result equals the same request served alone, bit
-for bit (the batched engine computes each instance exactly as the
-single-instance call does).
+Continuous batching at pass granularity: a scheduler thread keeps a queue
+of requests and a stepper (``FastGeodisCpp.Stepper``, the C ABI's
+``fg_stepper_*``) that holds the transforms in flight.  Each step advances
+every transform in flight by one pass pair, in launches shared by all of
+them; before each step the scheduler admits queued requests, in arrival
+order, while their estimated device memory fits the HBM budget, at most one
+new request per session per step.  So a request that arrives while others
+are on the GPU starts at the next step (a few hundred microseconds), and its
+sweeps share the launches of the requests already running; a request leaves
+with the step that runs its last pass.  Every request's result equals the
+same request served alone, bit for bit (a transform computes exactly the
+launches of its plain call, only in the company of others).
 
-Everything a batch does runs on the server's stream, in order: the click
-edits of a batch (one launch for all of its voxels) are written into the
-resident masks after the previous batch's transforms have read them, so the
-masks need no per-request copies.  Each session also keeps its image in the
-transposed layout the transform's W pass reads, so no click re-transposes it.
+Everything runs on the server's stream, in order: the click edits admitted
+before a step (one launch for all of their voxels) are written into the
+resident masks before that step, and a transform reads its mask only in its
+first step, so a session's next click may be admitted while its previous
+request is still running, and the masks need no per-request copies.  Each
+session keeps its image in the transposed layout the transform's W pass
+reads, so no click re-transposes it.  At most ``inflight`` steps are
+enqueued ahead of the device.
 
 Latency is measured from a request's arrival to the completion of the HIP
-event recorded after its batch.
+event recorded after the step that finished it.
 """
 from __future__ import annotations
 
@@ -72,8 +75,10 @@ class Click:
 class Result:
     """distance is complete when the future resolves.  It is recorded as in
     use on the stream that was current in the submitting thread (so freeing it
-    there is safe), and that stream waits for the batch that produced it; on
-    any other stream, call ``distance.record_stream(stream)`` before use."""
+    there is safe), and that stream waits for the step that produced it; on
+    any other stream, call ``distance.record_stream(stream)`` before use.
+    batch_size: the requests in flight (its own included) when it was
+    admitted."""
     distance: torch.Tensor         # (1, 1, D, H, W) signed distance, on device
     latency_s: float
     batch_size: int
@@ -83,11 +88,14 @@ class Server:
     """Continuous-batching server for click requests on resident sessions.
 
     budget_bytes: device memory the in-flight requests may use (outputs and
-    the batched call's workspace, estimated per request).
+    workspaces, estimated per request); max_batch: requests in flight at
+    once; inflight: steps enqueued ahead of the device; halo_bytes: the
+    stepper's halo-exchange buffer (a request that does not fit waits).
     """
 
     def __init__(self, budget_bytes: int = 8 << 30, max_batch: int = 64, v: float = 1e10,
-                 lamb: float = 1.0, iterations: int = 2, device: str = "cuda", inflight: int = 2):
+                 lamb: float = 1.0, iterations: int = 2, device: str = "cuda", inflight: int = 2,
+                 halo_bytes: int = 256 << 20):
         self.budget = int(budget_bytes)
         self.max_batch = int(max_batch)
         self.v, self.lamb, self.iterations = float(v), float(lamb), int(iterations)
@@ -105,7 +113,8 @@ class Server:
         self._thread: Optional[threading.Thread] = None
         self._error: Optional[BaseException] = None
         self._stream = torch.cuda.Stream(device=self.device)
-        self.batches: List[int] = []
+        self._stepper = FastGeodisCpp.Stepper(int(halo_bytes), 2 * self.max_batch, self.device.index)
+        self.batches: List[int] = []   # requests in flight at each step
         self.rejected = 0
 
     # ---- sessions -------------------------------------------------------
@@ -164,11 +173,11 @@ class Server:
             self._thread = None
 
     # ---- scheduler ------------------------------------------------------
-    def _admit(self) -> List[Tuple[Click, Future]]:
-        """Takes queued requests in arrival order while they fit the budget,
-        at most one per session."""
-        batch, used, seen, keep = [], 0, set(), collections.deque()
-        while self._queue and len(batch) < self.max_batch:
+    def _admit(self, used: int, inflight: int) -> List[Tuple[Click, Future]]:
+        """Takes queued requests in arrival order while they fit the budget and
+        the in-flight limit, at most one per session (this step)."""
+        batch, seen, keep = [], set(), collections.deque()
+        while self._queue and inflight + len(batch) < self.max_batch:
             click, fut = self._queue.popleft()
             need = self.request_bytes(click.session)
             if click.session in seen or used + need > self.budget:
@@ -184,31 +193,42 @@ class Server:
             self._queue.appendleft(keep.pop())
         return batch
 
-    def _launch(self, batch):
-        insts, masks, idx, vals = [], [], [], []
+    def _add(self, batch, inflight: int):
+        """Adds the admitted requests' transforms to the stepper and writes
+        their clicks' edits (on the server's stream, before the next step,
+        which is the first to read the masks).  Returns the requests taken and
+        those the stepper could not take yet (its exchange buffer is full)."""
         for click, _ in batch:
-            s = self.sessions[click.session]
-            _, _, D, H, W = s.mask.shape
-            val = 1.0 if click.foreground else 0.0
-            for (z, y, x) in click.voxels:
-                if not (0 <= z < D and 0 <= y < H and 0 <= x < W):
-                    raise ValueError(f"click voxel {(z, y, x)} outside session {click.session}")
-                masks.append(s.mask)
-                idx.append((z * H + y) * W + x)
-                vals.append(val)
-            # at most one click per session in a batch: the mask as edited is
-            # this request's view of it
-            insts.append(Instance(s.image, s.mask, self.v, self.lamb, iter=self.iterations,
-                                  spacing=list(s.spacing), op="signed", image_t=s.image_t))
+            shape = self.sessions[click.session].mask.shape[2:]
+            for vox in click.voxels:
+                if not all(0 <= c < n for c, n in zip(vox, shape)):
+                    raise ValueError(f"click voxel {tuple(vox)} outside session {click.session}")
+        taken, later = [], []
+        masks, idx, vals = [], [], []
         with torch.cuda.stream(self._stream):
-            FastGeodisCpp.set_voxels(masks, idx, vals)
-            outs = geodesic_batch(insts)
-            ev = torch.cuda.Event()
-            ev.record(self._stream)
-        return outs, ev
+            for click, fut in batch:
+                s = self.sessions[click.session]
+                ticket, out = self._stepper.add(s.image, s.mask, list(s.spacing), OP_SIGNED, self.v,
+                                                self.lamb, 1.0 - self.lamb, self.iterations, s.image_t)
+                if ticket < 0:
+                    later.append((click, fut))
+                    continue
+                taken.append((ticket, click, fut, out))
+                _, _, D, H, W = s.mask.shape
+                for (z, y, x) in click.voxels:
+                    masks.append(s.mask)
+                    idx.append((z * H + y) * W + x)
+                    vals.append(1.0 if click.foreground else 0.0)
+            if masks:
+                FastGeodisCpp.set_voxels(masks, idx, vals)
+        n = inflight + len(taken)
+        for ticket, click, fut, out in taken:
+            self._inflight[ticket] = (click, fut, out, n)
+        return taken, later
 
     def _run(self):
-        pending = collections.deque()  # (batch, outs, event)
+        pending = collections.deque()  # (event, tickets finished by that step)
+        self._inflight: Dict[int, tuple] = {}
         try:
             self._loop(pending)
         except BaseException as exc:  # fail every outstanding request instead of hanging it
@@ -216,47 +236,78 @@ class Server:
                 self._error = exc
                 waiting = [f for _, f in self._queue]
                 self._queue.clear()
-            for b, *_ in pending:
-                waiting.extend(f for _, f in b)
+            waiting.extend(f for _, f, _, _ in self._inflight.values())
             for f in waiting:
                 if not f.done():
                     f.set_exception(exc)
 
+    def _retire(self, pending, block: bool) -> None:
+        ev, done = pending[0]
+        if not ev.query():
+            if not block:
+                return
+            ev.synchronize()
+        pending.popleft()
+        t = time.perf_counter()
+        for ticket in done:
+            click, fut, out, n = self._inflight.pop(ticket)
+            consumer = click.stream
+            if consumer is not None and consumer != self._stream:
+                consumer.wait_event(ev)
+                out.record_stream(consumer)
+            fut.set_result(Result(out, t - click.arrival, n))
+
     def _loop(self, pending):
         torch.cuda.set_device(self.device)
+        used = 0      # budget bytes of the requests in the stepper
+        held = {}     # ticket -> its budget bytes
+        phase = 0     # pass (0 D, 1 H, 2 W) of the next step of every request in flight
         while True:
             with self._cv:
-                while not self._queue and not pending and not self._stop:
+                while not self._queue and not self._inflight and not pending and not self._stop:
                     self._cv.wait(timeout=0.05)
-                if self._stop and not self._queue and not pending:
+                if self._stop and not self._queue and not self._inflight and not pending:
                     return
-                batch = self._admit() if len(pending) < self.inflight_limit else []
+                active = self._stepper.active()
+                if active == 0:
+                    phase = 0
+                # requests join at iteration boundaries, so that every step
+                # runs one pass for all of them in shared launches
+                batch = self._admit(used, active) if len(pending) < self.inflight_limit and phase == 0 else []
             if batch:
                 try:
-                    outs, ev = self._launch(batch)
+                    taken, later = self._add(batch, active)
                 except BaseException as exc:
                     for _, f in batch:
                         f.set_exception(exc)
                     raise
-                pending.append((batch, outs, ev))
-                self.batches.append(len(batch))
+                for ticket, click, _, _ in taken:
+                    held[ticket] = self.request_bytes(click.session)
+                    used += held[ticket]
+                if later:
+                    with self._cv:
+                        for item in reversed(later):
+                            self._queue.appendleft(item)
+            if self._stepper.active() and len(pending) < self.inflight_limit:
+                with torch.cuda.stream(self._stream):
+                    done = self._stepper.step()
+                    ev = torch.cuda.Event()
+                    ev.record(self._stream)
+                pending.append((ev, done))
+                self.batches.append(self._stepper.active() + len(done))
+                phase = (phase + 1) % 3
+                for ticket in done:
+                    used -= held.pop(ticket)
                 continue
             if pending:
-                b, outs, ev = pending[0]
-                if not ev.query():
-                    # let new arrivals be admitted while the device is busy
+                # the device is busy: retire finished steps, or wait a little
+                # for new arrivals before blocking on the oldest step
+                before = len(pending)
+                self._retire(pending, block=False)
+                if len(pending) == before:
                     time.sleep(20e-6)
-                    if len(pending) < self.inflight_limit:
-                        continue
-                    ev.synchronize()
-                pending.popleft()
-                t = time.perf_counter()
-                for (click, fut), out in zip(b, outs):
-                    consumer = click.stream
-                    if consumer is not None and consumer != self._stream:
-                        consumer.wait_event(ev)
-                        out.record_stream(consumer)
-                    fut.set_result(Result(out, t - click.arrival, len(b)))
+                    if len(pending) >= self.inflight_limit or not self._stepper.active():
+                        self._retire(pending, block=True)
 
 
 def run_alone(server: Server, click: Click, mask_before: torch.Tensor) -> torch.Tensor:

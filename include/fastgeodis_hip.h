@@ -176,6 +176,36 @@ size_t fg_batch_workspace_bytes(const fg_instance *inst, int n);
 int fg_geodesic_batch_f32(const fg_instance *inst, int n, void *workspace,
                           size_t workspace_bytes, void *stream);
 
+/* Stepper: continuous batching at pass granularity (no counterpart in the
+ * reference; it serves the click-serving loop).  Unsigned and signed
+ * transforms (iterations >= 1, first sweep on the tiled engine) are added at
+ * any time; each fg_stepper_step records, for every active transform, its
+ * next pass pair (forward and backward sweep) in launches shared with every
+ * other active transform at that pass, plus the transposes around W passes,
+ * and enqueues them on `stream`.  A transform whose last W pass ran in a
+ * step is reported in `done` (its ticket) and has left the stepper; its
+ * output is complete when `stream` reaches the end of that step.  Results
+ * equal the same instance run alone.
+ *   halo: device memory for the halo exchange of all active transforms
+ *         (fg_stepper_op_bytes reports each transform's share; fg_stepper_add
+ *         refuses a transform that does not fit, FG_EWORKSPACE);
+ *   blob: device memory for one step's descriptors, fg_stepper_blob_bytes(
+ *         maximum active transforms; a signed instance counts 2);
+ *   workspace (per instance, fg_stepper_op_bytes): kept by the caller until
+ *         the instance is reported done, like its image, mask and output.
+ * The mask is read by the instance's first step only.  A stepper is used by
+ * one host thread at a time, on the device current at its creation. */
+typedef struct fg_stepper fg_stepper;
+size_t fg_stepper_blob_bytes(int max_transforms);
+size_t fg_stepper_op_bytes(const fg_instance *inst, size_t *halo_bytes);
+int fg_stepper_create(fg_stepper **out, void *halo, size_t halo_bytes, void *blob,
+                      size_t blob_bytes);
+int fg_stepper_add(fg_stepper *s, const fg_instance *inst, void *workspace,
+                   size_t workspace_bytes, int *ticket);
+int fg_stepper_step(fg_stepper *s, void *stream, int *done, int cap, int *ndone);
+int fg_stepper_active(const fg_stepper *s);
+void fg_stepper_destroy(fg_stepper *s);
+
 /* Writes n single float values (items[i].dst[items[i].index] = value),
  * asynchronously on `stream`, in one launch per 96 items; the items are
  * passed by value (no device allocation).  No counterpart in the reference:

@@ -42,7 +42,7 @@ def test_library_exports_every_declared_symbol(lib):
 
 def test_abi_version_and_errors(lib):
     lib.fg_abi_version.restype = ctypes.c_int
-    assert lib.fg_abi_version() == 3
+    assert lib.fg_abi_version() == 4
     lib.fg_last_error.restype = ctypes.c_char_p
     f = lib.fg_geodesic2d_f32
     f.restype = ctypes.c_int

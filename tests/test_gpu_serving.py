@@ -1,8 +1,8 @@
 """Serving loop: continuous batching of click requests on resident sessions.
 
 Every request's result must equal the same request served alone (bit for
-bit), whatever shared its batch; requests for one session keep their order;
-the HBM budget caps what is admitted at once.
+bit), whatever shared its steps; requests for one session keep their order;
+the HBM budget caps what is in flight at once.
 """
 import threading
 import time
@@ -62,8 +62,8 @@ def test_budget_limits_batches(device):
         res = [f.result(timeout=60) for f in futs]
     finally:
         srv.stop()
-    assert max(srv.batches) <= 3
-    assert sum(srv.batches) == 8
+    assert max(srv.batches) <= 3  # requests in flight at each step
+    assert len(res) == 8 and max(r.batch_size for r in res) <= 3
     assert all(r.latency_s > 0 for r in res)
 
 
