@@ -76,6 +76,16 @@ constexpr int kRowPollDelay = 8;
 constexpr bool kSkipEdgeLoads = true;
 // Waves that perform the halo exchanges in kernels with exchange waves.
 constexpr int kCommWaves = 4;
+// A-contiguous output (the W pass writing the native layout): slices staged
+// per run (16 x 4 B = one 64-byte segment per position; measured in the
+// sweep's streaming pattern: runs of 16 cost ~10 % more than plain row
+// stores, runs of 8 ~50 %) and the padding between run slots in LDS (words).
+constexpr int kRunLen = 16;
+constexpr int kRunPad = 4;
+// A completed run is stored at the next halo exchange, so the LDS ring also
+// holds the slices staged until then (K - 1 at most, the host using this
+// form with K == 5).
+constexpr int kRunSlots = kRunLen + 4;
 // Buckets of the exchange-wait histogram: bucket b counts waits of
 // [2^(b-1), 2^b) ticks of the constant-rate device clock (bucket 0: 0 ticks).
 constexpr int kXHistBuckets = 32;
@@ -98,8 +108,9 @@ struct SweepDesc {
   int TB, TE;    // own tile extent
   int tile0;     // first tile of this descriptor within its launch
   long long halo0;  // first halo entry of this descriptor
-  float *dxo;       // transposed-output kernels: output array, element (a, b, e) at
-  long long sXe, sXa;  //   dxo[e * sXe + a * sXa + b] (b contiguous)
+  float *dxo;       // other-layout output: element (a, b, e) at
+  long long sXe, sXa;  //   dxo[e * sXe + a * sXa + b] (XST, b contiguous)
+  long long sXb;       //   dxo[e * sXe + b * sXb + a] (XSA, a contiguous)
   // First sweep of a transform: the input distance is the initialisation,
   // computed from the mask as it is loaded (init_mode 1: fl(v*m), 2:
   // fl(v*fl(1-m))), and the sweep's untouched first slice is written too.
@@ -266,10 +277,13 @@ struct Slab {  // one prefetched slice: image rows -1..R, distance rows 0..R-1
 
 // NW waves per workgroup, R window rows per wave (register rows), P slices
 // prefetched ahead, CT image channels.
-// XST: instead of writing slice a in the sweep's layout, write it to Dd.dxo
-// with B contiguous (the H pass writes the (W, H, D) layout of the W pass),
-// staging each slice's window through LDS so each store covers a run of
-// consecutive B.
+// XS, the output form: 0 writes slice a in the sweep's own layout.
+//   1 (XST): writes it to Dd.dxo with B contiguous (the H pass writes the
+//     (W, H, D) layout of the W pass), staging each slice's window through
+//     LDS so each store covers a run of consecutive B.
+//   2 (XSA): writes it to Dd.dxo with A contiguous (the W pass writes the
+//     native (D, H, W) layout), staging kRunLen consecutive slices per wave
+//     in LDS so that each store covers runs of kRunLen consecutive A.
 // INIT: the first sweep of a transform, whose input distance is computed from
 // the mask as it is loaded (SweepDesc::init_mode).
 // COMM: kCommWaves more waves (indices NW, NW + 1, ...) that hold no rows and
@@ -278,7 +292,7 @@ struct Slab {  // one prefetched slice: image rows -1..R, distance rows 0..R-1
 // they publish and poll, so the compute waves' prefetch loads stay in flight
 // (a wave's wait for a halo load is also a wait for every load it issued
 // earlier).
-template <int NW, int R, int P, int CT, bool LAMB1, bool FAST, bool XST = false, bool INIT = false,
+template <int NW, int R, int P, int CT, bool LAMB1, bool FAST, int XS = 0, bool INIT = false,
           bool COMM = false>
 __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_tiled_kernel(
     const SweepDesc *__restrict__ descs, int ndesc, SweepDesc one, int K, unsigned tag_base,
@@ -287,16 +301,29 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   // LDS slot = ring position of the step (a compile-time index in the
   // unrolled loops, so edge addresses are immediate offsets)
   constexpr int NWL = NW > 1 ? NW : 1;
-  __shared__ float edge_first[P][NWL][64];
-  __shared__ float edge_last[P][NWL][64];
-  __shared__ float edgeI_first[P][NWL][CT][64];
-  __shared__ float edgeI_last[P][NWL][CT][64];
+  constexpr bool XST = XS == 1, XSA = XS == 2;
+  // XSA needs the LDS for its runs: two edge slots, alternating by step
+  // parity (one barrier per step separates a slot's reads from its rewrite
+  // two steps later)
+  constexpr int ES = XSA ? 2 : P;
+  __shared__ float edge_first[ES][NWL][64];
+  __shared__ float edge_last[ES][NWL][64];
+  __shared__ float edgeI_first[ES][NWL][CT][64];
+  __shared__ float edgeI_last[ES][NWL][CT][64];
   constexpr int NWR = NW * R;  // window rows
   // one wave of one row: the plane is a single row (nB == 1), so the only
   // neighbours are lanes -1, 0, +1 of the front row itself
   constexpr bool ROW1 = NW == 1 && R == 1;
   static_assert(!XST || (NW > 1 && 64 % NWR == 0), "transposed output needs a multi-wave window of 16/32/64 rows");
+  static_assert(!XSA || (NW > 1 && !COMM && !INIT && NWR > 2 * P), "A-contiguous output is a multi-wave, plain-sweep form");
   __shared__ float xtile[XST ? P : 1][XST ? NWR : 1][65];
+  // XSA: slot a % kRunSlots holds slice a, one 64-lane row per own row of
+  // the tile (window rows K .. K + TB - 1; the host uses this form only on
+  // planes of several tiles along B, where TB <= NWR - 2K); the slots are
+  // kRunPad words apart beyond their rows, so the 16 lanes that read one
+  // position's run hit different banks
+  constexpr int XOR = NWR - 2 * P;  // own rows at the shallowest halo
+  __shared__ float xrun[XSA ? kRunSlots : 1][XSA ? XOR * 64 + kRunPad : 1];
   __shared__ unsigned xhist_wg[kXHistBuckets];  // this workgroup's exchange-wait counts (measurement)
   static_assert(!COMM || (NW > 1 && NWR % kCommWaves == 0), "exchange waves serve multi-wave sweeps");
   __shared__ float xfront[COMM ? NWR : 1][64];     // COMM: the front around an exchange
@@ -353,8 +380,11 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
 
   // The block loop below advances in groups of P steps and exchanges halos
   // every K steps; it stays within the sweep only when K is a positive
-  // multiple of P.  A launch that breaks this is refused before any store.
-  if (K <= 0 || K % P != 0 || (ROW1 && Dd.nB != 1)) {
+  // multiple of P.  A launch that breaks this is refused before any store,
+  // and so is an A-contiguous output whose own rows or staged slices would
+  // not fit its LDS ring.
+  if (K <= 0 || K % P != 0 || (ROW1 && Dd.nB != 1) ||
+      (XSA && (Dd.nTB < 2 || Dd.TB > XOR || K > kRunSlots - kRunLen + 1))) {
     if (threadIdx.x == 0) __hip_atomic_store(status, FG_EINVAL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     return;
   }
@@ -616,6 +646,74 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     row_own[r] = rb[r + 1] >= b_own0 && rb[r + 1] < b_own1;
   }
 
+  // XSA: stage the wave's own-row values of slice a in LDS slot a % kRunSlots.
+  // Runs are the kRunLen slices a0 .. a0 + kRunLen - 1 with a0 a multiple of
+  // kRunLen (clipped to the sweep).  A run whose slices have all been staged
+  // is stored at the next halo exchange, between its halo loads and the
+  // prefetch, where the wave waits for its earlier memory operations anyway
+  // (stores issued inside the steps made the compiler's waits for the
+  // prefetched slabs conservative on every path); the K <= kRunSlots -
+  // kRunLen slices staged before then fill free slots.  Lane l moves slot
+  // a0 + l % 16 of window lane 4j + l / 16, for j < 16, so every 16 lanes
+  // store one 64-byte segment of the output.  Only this wave reads what it
+  // staged and a wave's LDS accesses complete in order, so no barrier is
+  // needed; the wave-scope fences keep the compiler from moving the accesses
+  // across each other.  Slices past the end of the array (a partial first or
+  // last run) and positions the tile does not own get an out-of-range
+  // offset.  The host uses this form only for outputs of less than 4 GiB,
+  // so 32-bit offsets address every element (the wrapped products of a
+  // negative window lane are replaced before use).
+  int xa_a0 = -1;  // first slice of the run awaiting its stores, or -1
+  auto xa_flush = [&]() {
+    if constexpr (XSA) {
+      constexpr int LPR = kRunLen, PPI = 64 / kRunLen;
+      static_assert(64 % kRunLen == 0, "run layout");
+      if (xa_a0 >= 0) {
+        const int q = lane % LPR, pl = lane / LPR;
+        const int sq = (xa_a0 + q) % kRunSlots;  // LDS slot of this lane's slice
+        const unsigned sXe4 = (unsigned)Dd.sXe * 4u;
+        const unsigned lane_base = (unsigned)(ew0 + pl) * sXe4 + (unsigned)(xa_a0 + q) * 4u;
+        const bool a_ok = xa_a0 + q < nA;
+        const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc((void *)Dd.dxo, 0, 0xFFFFFFF0u, 0x00020000);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          if (!row_own[r]) continue;
+          const unsigned row_base = (unsigned)rb[r + 1] * (unsigned)Dd.sXb * 4u;
+#pragma unroll
+          for (int j = 0; j < kRunLen; ++j) {
+            const int p = j * PPI + pl;  // window lane
+            const int ep = ew0 + p;
+            const bool own = a_ok && ep >= e_own0 && ep < e_own1;
+            const float x = xrun[sq][(w * R + r - K) * 64 + p];
+            const unsigned off = own ? lane_base + (unsigned)(j * PPI) * sXe4 + row_base : 0xFFFFFFFFu;
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), rx, (int)off, 0, 0);
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        xa_a0 = -1;
+      }
+    }
+  };
+  auto xa_stage = [&](const float (&v)[R], int a) {
+    if constexpr (XSA) {
+      if ((unsigned)a < (unsigned)nA) {
+        const int sa = a % kRunSlots;
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+          if (row_own[r]) xrun[sa][(w * R + r - K) * 64 + lane] = v[r];
+        // one run completes per block at most (K < kRunLen); a forward
+        // sweep's partial last run is stored after the loop
+        const int g = a & (kRunLen - 1);
+        if (dir < 0 ? g == 0 : g == kRunLen - 1) xa_a0 = a - g;
+      }
+    }
+  };
+
   // Slab ring: the slab of step s is S[s % P]; its image rows are also the
   // front image of step s + 1, so a slab is refilled (with the slice of step
   // s + P) only after step s + 1 has used it; the front of step 0 starts in
@@ -639,6 +737,7 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
 #pragma unroll
     for (int r = 0; r < R; ++r) xtile[P - 1][w * R + r][lane] = Fd[r];
   }
+  xa_stage(Fd, a_first - dir);  // XSA: the sweep's untouched first slice
 #pragma unroll
   for (int p = 0; p < P - 1; ++p) load_slab(S[p], cursor_of(p), p);
   // the slices that the loop stores (step s) and prefetches (step s + P - 1)
@@ -718,6 +817,7 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
       x[r] = 0;
       if (need[r]) x[r] = __hip_atomic_load(src[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    xa_flush();  // XSA: a run completed in this block
     load_slab(pf_slab, cur_pf, P - 2);
     advance(cur_pf);
     // after an expiry this wave waits no more: the sweep then finishes
@@ -762,22 +862,23 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     // the distance; their outer image rows then only meet +inf candidates)
     float top = kInf, bot = kInf;
     if constexpr (NW > 1) {
-      edge_first[slot][w][lane] = Fd[0];
-      edge_last[slot][w][lane] = Fd[R - 1];
+      const int es = XSA ? (s & 1) : slot;
+      edge_first[es][w][lane] = Fd[0];
+      edge_last[es][w][lane] = Fd[R - 1];
 #pragma unroll
       for (int c = 0; c < CT; ++c) {
-        edgeI_first[slot][w][c][lane] = Sl.I[c][1];
-        edgeI_last[slot][w][c][lane] = Sl.I[c][R];
+        edgeI_first[es][w][c][lane] = Sl.I[c][1];
+        edgeI_last[es][w][c][lane] = Sl.I[c][R];
       }
       __syncthreads();
-      const float t = edge_last[slot][max(w - 1, 0)][lane];
-      const float u = edge_first[slot][min(w + 1, NW - 1)][lane];
+      const float t = edge_last[es][max(w - 1, 0)][lane];
+      const float u = edge_first[es][min(w + 1, NW - 1)][lane];
       top = (w > 0) ? t : kInf;
       bot = (w < NW - 1) ? u : kInf;
 #pragma unroll
       for (int c = 0; c < CT; ++c) {
-        Sl.I[c][0] = edgeI_last[slot][max(w - 1, 0)][c][lane];
-        Sl.I[c][R + 1] = edgeI_first[slot][min(w + 1, NW - 1)][c][lane];
+        Sl.I[c][0] = edgeI_last[es][max(w - 1, 0)][c][lane];
+        Sl.I[c][R + 1] = edgeI_first[es][min(w + 1, NW - 1)][c][lane];
       }
       // the previous step's staged window is complete (barrier above)
       xflush((slot + P - 1) % P, s - 1);
@@ -815,6 +916,8 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     if constexpr (XST) {
 #pragma unroll
       for (int r = 0; r < R; ++r) xtile[slot][w * R + r][lane] = nd[r];
+    } else if constexpr (XSA) {
+      xa_stage(nd, cur_st.a);
     } else {
       // a padded step's slice is outside the array: all its stores are dropped
       const __amdgpu_buffer_rsrc_t rs = cursor_rsrc(Dd.d, cur_st);
@@ -861,6 +964,13 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   if constexpr (XST) {
     __syncthreads();
     xflush(P - 1, nblocks * K - 1);  // the last step sits in slot P - 1
+  }
+  if constexpr (XSA) {
+    xa_flush();  // a run completed in the last block
+    if (dir > 0 && ((nA - 1) & (kRunLen - 1)) != kRunLen - 1) {  // a forward sweep's partial last run
+      xa_a0 = (nA - 1) & ~(kRunLen - 1);
+      xa_flush();
+    }
   }
   if (xhist) {
     __syncthreads();
@@ -1225,6 +1335,7 @@ struct Xform {
   int iters;
   int it = 0;  // the iteration being recorded
   bool xst;  // this iteration's H-backward sweep writes dT directly (no transpose before the W pass)
+  bool xsa = false;  // this iteration's W-backward sweep writes d directly (no transpose after the W pass)
   // The first sweep initialises the distance from the mask (init_mode 1: v*m,
   // 2: v*(1-m)) instead of reading it; set only when that sweep is tiled.
   int init_mode = 0;
@@ -1338,14 +1449,14 @@ void pass_plane(const Xform &x, int pass, int &nA, int &nB, int &nE) {
   else { nA = x.W; nB = x.H; nE = x.D; }
 }
 
-template <int NW, int R, int P, int CT, bool L1, bool FA, bool XS = false, bool IN = false, bool CM = false>
+template <int NW, int R, int P, int CT, bool L1, bool FA, int XS = 0, bool IN = false, bool CM = false>
 const void *kernel_ptr() {
   return reinterpret_cast<const void *>(&sweep_tiled_kernel<NW, R, P, CT, L1, FA, XS, IN, CM>);
 }
 
 // lamb1 implies fast (l_grad == 1 and l_eucl*LD == +0 are non-negative).
 // Only the channel counts a (shape, ring) pair serves are instantiated.
-template <int NW, int R, int P, int CT, bool XS = false, bool IN = false, bool CM = false>
+template <int NW, int R, int P, int CT, int XS = 0, bool IN = false, bool CM = false>
 const void *pick_variant(bool lamb1, bool fast) {
   if (lamb1) return kernel_ptr<NW, R, P, CT, true, true, XS, IN, CM>();
   return fast ? kernel_ptr<NW, R, P, CT, false, true, XS, IN, CM>()
@@ -1354,20 +1465,21 @@ const void *pick_variant(bool lamb1, bool fast) {
 
 template <int NW, int R, int P, int CT>
 const void *pick_variant_init(bool lamb1, bool fast, bool init) {
-  return init ? pick_variant<NW, R, P, CT, false, true>(lamb1, fast) : pick_variant<NW, R, P, CT>(lamb1, fast);
+  return init ? pick_variant<NW, R, P, CT, 0, true>(lamb1, fast) : pick_variant<NW, R, P, CT>(lamb1, fast);
 }
 
 // Kernels of the 8 x 4 shape with an exchange wave (ring 5, 1-2 channels).
-const void *pick_comm_kernel(int C, bool lamb1, bool fast, bool xst, bool init) {
+const void *pick_comm_kernel(int C, bool lamb1, bool fast, int xs, bool init) {
+  if (xs == 2) return nullptr;
   if (C == 1) {
-    if (xst) return pick_variant<8, 4, 5, 1, true, false, true>(lamb1, fast);
-    return init ? pick_variant<8, 4, 5, 1, false, true, true>(lamb1, fast)
-                : pick_variant<8, 4, 5, 1, false, false, true>(lamb1, fast);
+    if (xs == 1) return pick_variant<8, 4, 5, 1, 1, false, true>(lamb1, fast);
+    return init ? pick_variant<8, 4, 5, 1, 0, true, true>(lamb1, fast)
+                : pick_variant<8, 4, 5, 1, 0, false, true>(lamb1, fast);
   }
   if (C == 2) {
-    if (xst) return pick_variant<8, 4, 5, 2, true, false, true>(lamb1, fast);
-    return init ? pick_variant<8, 4, 5, 2, false, true, true>(lamb1, fast)
-                : pick_variant<8, 4, 5, 2, false, false, true>(lamb1, fast);
+    if (xs == 1) return pick_variant<8, 4, 5, 2, 1, false, true>(lamb1, fast);
+    return init ? pick_variant<8, 4, 5, 2, 0, true, true>(lamb1, fast)
+                : pick_variant<8, 4, 5, 2, 0, false, true>(lamb1, fast);
   }
   return nullptr;
 }
@@ -1386,17 +1498,26 @@ const void *pick_kernel_nr(int C, bool lamb1, bool fast, bool init) {
 }
 
 // The kernel compiled for exactly this (NW, R, P) and channel count, or null.
-// Transposed-output kernels exist for the two 5-slice-ring 3D shapes: 8 x 4
-// with 1-2 channels, 16 x 2 with 1 channel (2 channels would exceed the LDS).
+// xs selects the output form (0 own layout, 1 B contiguous, 2 A contiguous).
+// B-contiguous kernels exist for the two 5-slice-ring 3D shapes: 8 x 4 with
+// 1-2 channels, 16 x 2 with 1 channel (2 channels would exceed the LDS);
+// A-contiguous ones for both shapes with 1 channel.
 constexpr int kXstMaxC = 2;
-const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast, bool xst = false, bool init = false,
+constexpr int kXsaMaxC = 1;
+constexpr int kXsaMaxK = kRunSlots - kRunLen + 1;
+const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast, int xs = 0, bool init = false,
                         bool comm = false) {
-  if (xst && init) return nullptr;
-  if (comm) return (NW == 8 && R == 4 && P == 5) ? pick_comm_kernel(C, lamb1, fast, xst, init) : nullptr;
-  if (xst) {
-    if (NW == 8 && R == 4 && P == 5 && C == 1) return pick_variant<8, 4, 5, 1, true>(lamb1, fast);
-    if (NW == 8 && R == 4 && P == 5 && C == 2) return pick_variant<8, 4, 5, 2, true>(lamb1, fast);
-    if (NW == 16 && R == 2 && P == 5 && C == 1) return pick_variant<16, 2, 5, 1, true>(lamb1, fast);
+  if (xs && init) return nullptr;
+  if (comm) return (NW == 8 && R == 4 && P == 5) ? pick_comm_kernel(C, lamb1, fast, xs, init) : nullptr;
+  if (xs == 1) {
+    if (NW == 8 && R == 4 && P == 5 && C == 1) return pick_variant<8, 4, 5, 1, 1>(lamb1, fast);
+    if (NW == 8 && R == 4 && P == 5 && C == 2) return pick_variant<8, 4, 5, 2, 1>(lamb1, fast);
+    if (NW == 16 && R == 2 && P == 5 && C == 1) return pick_variant<16, 2, 5, 1, 1>(lamb1, fast);
+    return nullptr;
+  }
+  if (xs == 2) {
+    if (NW == 8 && R == 4 && P == 5 && C == 1) return pick_variant<8, 4, 5, 1, 2>(lamb1, fast);
+    if (NW == 16 && R == 2 && P == 5 && C == 1) return pick_variant<16, 2, 5, 1, 2>(lamb1, fast);
     return nullptr;
   }
 #define FG_S(nw, r, p, clo, chi) \
@@ -1574,8 +1695,9 @@ int status_error(int st, const char *when) {
                                   "compute units); the results of that call are invalid");
   if (st == FG_EINVAL)
     return fail(FG_EHIP, std::string(when) +
-                             ": a sweep launch was refused (halo depth not a multiple of its ring, or a one-row "
-                             "kernel on a plane of several rows)");
+                             ": a sweep launch was refused (halo depth not a multiple of its ring, a one-row "
+                             "kernel on a plane of several rows, or a native-layout output on a plane of one "
+                             "tile row)");
   return fail(FG_EHIP, std::string(when) + ": device status " + std::to_string(st));
 }
 
@@ -1698,7 +1820,7 @@ struct Runner {
 
   bool no_graph = false;  // replay the launches directly (sequences that do not repeat)
 
-  void fill_desc(const Xform &x, int pass, int dir, SweepDesc &s, bool xst = false) {
+  void fill_desc(const Xform &x, int pass, int dir, SweepDesc &s, int xs = 0) {
     std::memset(&s, 0, sizeof s);
     if (x.init_mode && x.it == 0 && pass == first_pass(x) && dir > 0) {
       s.init_src = x.init_src;
@@ -1723,10 +1845,14 @@ struct Runner {
       s.img = x.imgT; s.d = x.dT; s.sA = (long long)x.H * x.D; s.sB = x.D;
     }
     for (int k = 0; k < 9; ++k) s.e1[k] = x.e1[pass][k];
-    if (xst) {  // H pass (A = H, B = D, E = W) written as (W, H, D)
+    if (xs == 1) {  // H pass (A = H, B = D, E = W) written as (W, H, D)
       s.dxo = x.dT;
       s.sXe = (long long)s.nA * s.nB;
       s.sXa = s.nB;
+    } else if (xs == 2) {  // W pass (A = W, B = H, E = D) written as (D, H, W)
+      s.dxo = x.d;
+      s.sXe = (long long)x.H * x.W;
+      s.sXb = x.W;
     }
   }
 
@@ -1740,9 +1866,25 @@ struct Runner {
     pass_plane(x, 1, nA, nB, nE);
     if (nA <= 1 || nB <= 1) return false;
     const Plan p = plan_plane(nB, nE, K, x.C, dev.cus);
-    if (!p.ok || !pick_kernel(p.NW, p.R, p.P, x.C, x.lamb1, x.fast, true, false, p.comm)) return false;
+    if (!p.ok || !pick_kernel(p.NW, p.R, p.P, x.C, x.lamb1, x.fast, 1, false, p.comm)) return false;
     if ((long long)x.D * x.H * x.W * 4 >= 0xFFFFFFF0ll) return false;
     return slice_offsets_fit(x, 1);
+  }
+
+  // Whether the W-backward sweep of this transform may write the native
+  // layout itself (the launch then also needs a plan of several tile rows
+  // and an A-contiguous kernel for its shape, checked when it is recorded):
+  // 3D, tiled engine, one channel, output offsets within 32 bits, and not
+  // the last iteration of a transform whose result leaves through a
+  // combining transpose (signed and GSF epilogues).
+  bool xsa_candidate(const Xform &x) {
+    if (x.is2d || engine == 0 || x.C > kXsaMaxC) return false;
+    if (x.it == x.iters - 1 && (x.fin_member || x.fin_kind != XK_COPY)) return false;
+    int nA, nB, nE;
+    pass_plane(x, 2, nA, nB, nE);
+    if (nA <= 1 || nB <= 1) return false;
+    if ((long long)x.D * x.H * x.W * 4 >= 0xFFFFFFF0ll) return false;
+    return slice_offsets_fit(x, 2);
   }
 
   // The tiled kernel addresses elements of a slice with 32-bit byte offsets
@@ -1776,7 +1918,7 @@ struct Runner {
   }
 
   int record_tiled(std::vector<SweepJob> &jobs, int dir, int NW, int R, int P, int C, bool lamb1, bool fast,
-                   bool xs = false, bool in = false, bool cm = false) {
+                   int xs = 0, bool in = false, bool cm = false) {
     std::vector<SweepDesc> descs;
     long long halo_used = 0;
     int tiles = 0, max_blocks = 0, kk = -1;
@@ -1960,7 +2102,7 @@ struct Runner {
   int sweep_pass(std::vector<Xform *> &active, int pass) {
     const std::vector<int> ks = pass_halos(active, pass);
     for (int dir : {+1, -1}) {
-      struct G { int NW, R, P, K, C; bool l1, fa, xs, in, cm; std::vector<SweepJob> jobs; };
+      struct G { int NW, R, P, K, C; bool l1, fa; int xs; bool in, cm; std::vector<SweepJob> jobs; };
       std::vector<G> groups;
       for (size_t i = 0; i < active.size(); ++i) {
         Xform *x = active[i];
@@ -1971,10 +2113,19 @@ struct Runner {
         if (ks[i] == 0) {
           if (in) return fail(FG_EINVAL, "internal: an initialising sweep was planned on the per-slice engine");
           record_per_slice(*x, pass, dir);
+          if (pass == 2) x->xsa = false;  // its output stays in the W pass layout
           continue;
         }
         const Plan p = plan_fixed(nB, nE, ks[i], x->C);
-        const bool xs = pass == 1 && dir < 0 && x->xst;
+        int xs = (pass == 1 && dir < 0 && x->xst) ? 1 : 0;
+        if (pass == 2 && dir < 0 && x->xsa) {
+          // the A-contiguous form stages the own rows of a tile of several
+          // along B, and the slices of one block beyond a run; otherwise the
+          // distance leaves through a transpose
+          if (p.nTB > 1 && p.K <= kXsaMaxK && pick_kernel(p.NW, p.R, p.P, x->C, x->lamb1, x->fast, 2, in, p.comm))
+            xs = 2;
+          else x->xsa = false;
+        }
         bool found = false;
         for (auto &g : groups)
           if (g.NW == p.NW && g.R == p.R && g.P == p.P && g.K == p.K && g.C == x->C && g.l1 == x->lamb1 &&
@@ -1995,7 +2146,7 @@ struct Runner {
 
   // A group is split into launches whose tiles can all be co-resident.
   int record_group(std::vector<SweepJob> &jobs, int dir, int NW, int R, int P, int C, bool l1, bool fa,
-                   bool xs = false, bool in = false, bool cm = false) {
+                   int xs = 0, bool in = false, bool cm = false) {
     const void *kern = pick_kernel(NW, R, P, C, l1, fa, xs, in, cm);
     if (!kern) return fail(FG_EINVAL, "no tiled kernel for this shape");
     long long cap = 0;
@@ -2020,7 +2171,8 @@ struct Runner {
       if (nt > cap) {  // one plane alone cannot be co-resident: per-slice engine
         if (in) return fail(FG_EINVAL, "internal: an initialising sweep was planned on the per-slice engine");
         record_per_slice(*j.x, j.pass, dir);
-        j.x->xst = false;  // its output stays in the native layout
+        if (j.pass == 1) j.x->xst = false;  // its output stays in the native layout
+        if (j.pass == 2) j.x->xsa = false;  // its output stays in the W pass layout
         continue;
       }
       if (tiles + nt > cap) {
@@ -2036,14 +2188,22 @@ struct Runner {
   }
 
   // The transposes around a W pass: into its layout (unless the H pass's
-  // backward sweep wrote it) and back, the last iteration's carrying the
-  // group's epilogue (members of a group contribute through its first one).
-  void w_pass_transposes(const std::vector<Xform *> &active, std::vector<XposeDesc> &to,
-                         std::vector<XposeDesc> &back) {
+  // backward sweep wrote it) and, once the W pass is recorded, back (unless
+  // its backward sweep wrote the native layout), the last iteration's
+  // carrying the group's epilogue (members of a group contribute through its
+  // first one).
+  void w_pass_to(const std::vector<Xform *> &active, std::vector<XposeDesc> &to) {
     for (auto *x : active) {
       int D, H, W;
       native_dims(*x, D, H, W);
       if (!x->xst) to.push_back(xd(x->d, x->dT, 1, D, H, W));
+    }
+  }
+  void w_pass_back(const std::vector<Xform *> &active, std::vector<XposeDesc> &back) {
+    for (auto *x : active) {
+      if (x->xsa) continue;
+      int D, H, W;
+      native_dims(*x, D, H, W);
       const bool last = x->it == x->iters - 1;
       if (last && x->fin_member) continue;  // part of its group's combined transpose
       XposeDesc b = xd(x->dT, x->d, 1, W, H, D);
@@ -2074,13 +2234,17 @@ struct Runner {
           active.push_back(&x);
         }
       int rc;
-      for (auto *x : active) x->xst = xstore_candidate(*x);
+      for (auto *x : active) {
+        x->xst = xstore_candidate(*x);
+        x->xsa = xsa_candidate(*x);
+      }
       if ((rc = sweep_pass(active, 0))) return rc;
       if ((rc = sweep_pass(active, 1))) return rc;
       std::vector<XposeDesc> to, back;
-      w_pass_transposes(active, to, back);
+      w_pass_to(active, to);
       record_xpose(to);
       if ((rc = sweep_pass(active, 2))) return rc;
+      w_pass_back(active, back);
       record_xpose(back);
     }
     return FG_OK;
@@ -2798,11 +2962,13 @@ int stepper_step(Stepper &st, hipStream_t stream, int *done, int cap, int *ndone
       act[op.pass].push_back(&x);
     }
   for (auto *x : act[1]) x->xst = run.xstore_candidate(*x);
+  for (auto *x : act[2]) x->xsa = run.xsa_candidate(*x);
   std::vector<XposeDesc> to, back;
-  run.w_pass_transposes(act[2], to, back);
+  run.w_pass_to(act[2], to);
   run.record_xpose(to);
   for (int pass = 0; pass < 3; ++pass)
     if (!act[pass].empty() && (rc = run.sweep_pass(act[pass], pass))) return rc;
+  run.w_pass_back(act[2], back);
   run.record_xpose(back);
   if ((rc = run.replay(stream, st.blob, st.blob_cap, st.halo, dstate.status_dev, false))) return rc;
   st.halo_clear = true;

@@ -124,6 +124,11 @@ RANDOM_3D = [
     ((1, 1, 1), [1.0, 1.0, 1.0], 1, 0.5, 2),
     ((3, 200, 2), [1.0, 0.5, 2.0], 1, 0.7, 2),
     ((64, 2, 300), [1.5, 1.0, 1.0], 3, 1.0, 2),
+    # W-backward sweeps writing the native layout in runs of 16 slices: a
+    # partial first run (W - 1 = 16, 32), W below one run, several B tiles
+    ((90, 33, 17), [1.0, 1.3, 0.8], 1, 1.0, 2),
+    ((70, 66, 33), [2.0, 1.0, 1.0], 1, 0.5, 3),
+    ((130, 40, 9), [1.0, 1.0, 1.0], 1, 0.0, 2),
 ]
 
 
