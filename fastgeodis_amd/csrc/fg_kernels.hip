@@ -649,10 +649,10 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   // XSA: stage the wave's own-row values of slice a in LDS slot a % kRunSlots.
   // Runs are the kRunLen slices a0 .. a0 + kRunLen - 1 with a0 a multiple of
   // kRunLen (clipped to the sweep).  A run whose slices have all been staged
-  // is stored at the next halo exchange, between its halo loads and the
-  // prefetch, where the wave waits for its earlier memory operations anyway
-  // (stores issued inside the steps made the compiler's waits for the
-  // prefetched slabs conservative on every path); the K <= kRunSlots -
+  // is stored at the next halo exchange, after the tile's publications,
+  // where the wave waits for its earlier memory operations anyway (stores
+  // issued inside the steps made the compiler's waits for the prefetched
+  // slabs conservative on every path); the K <= kRunSlots -
   // kRunLen slices staged before then fill free slots.  Lane l moves slot
   // a0 + l % 16 of window lane 4j + l / 16, for j < 16, so every 16 lanes
   // store one 64-byte segment of the output.  Only this wave reads what it
@@ -792,6 +792,11 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
         __hip_atomic_store(pub + (long long)(b - b_own0) * own_w + lane_pub, pack(Fd[r], tag), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
+    // XSA: a run completed in this block is stored after the publications
+    // (which then leave first) and drains while the wave waits to poll; the
+    // halo wait below also waits for it, so every path leaves the exchange
+    // with the same loads in flight
+    xa_flush();
     // Neighbours publish at about the same moment; polling right away mostly
     // finds their values not yet visible and costs a second round trip.
     // Waiting ~1500 clocks first was measured fastest for multi-wave tiles
@@ -817,7 +822,6 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
       x[r] = 0;
       if (need[r]) x[r] = __hip_atomic_load(src[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
-    xa_flush();  // XSA: a run completed in this block
     load_slab(pf_slab, cur_pf, P - 2);
     advance(cur_pf);
     // after an expiry this wave waits no more: the sweep then finishes
