@@ -282,7 +282,8 @@ def main():
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "sweep_tiled_kernel<16 waves, 2 rows, 5-slice ring, C=1, "
-                          + ("lamb=1" if args.lamb == 1.0 else "general lamb") + "> (plain and transposed-output forms)",
+                          + ("lamb=1" if args.lamb == 1.0 else "general lamb")
+                          + "> (own-layout, H-backward B-contiguous and W-backward A-contiguous output forms)",
                 "launches": launches,
                 "mean_launch_ms": round(avg_ms, 4), "alg_bytes_per_launch": bytes_per_launch}
     whole_gbs = updates * 12 / (ms_per_step * 1e-3) / 1e9

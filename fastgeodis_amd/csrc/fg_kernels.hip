@@ -322,8 +322,8 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   // planes of several tiles along B, where TB <= NWR - 2K); the slots are
   // kRunPad words apart beyond their rows, so the 16 lanes that read one
   // position's run hit different banks
-  constexpr int XOR = NWR - 2 * P;  // own rows at the shallowest halo
-  __shared__ float xrun[XSA ? kRunSlots : 1][XSA ? XOR * 64 + kRunPad : 1];
+  constexpr int XOR = NWR - 2 * P;  // own rows at the shallowest halo (+ 1 spare row)
+  __shared__ float xrun[XSA ? kRunSlots : 1][XSA ? (XOR + 1) * 64 + kRunPad : 1];
   __shared__ unsigned xhist_wg[kXHistBuckets];  // this workgroup's exchange-wait counts (measurement)
   static_assert(!COMM || (NW > 1 && NWR % kCommWaves == 0), "exchange waves serve multi-wave sweeps");
   __shared__ float xfront[COMM ? NWR : 1][64];     // COMM: the front around an exchange
@@ -699,18 +699,23 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
       }
     }
   };
+  // Branch-free staging (a branch in the step keeps the compiler from
+  // moving work across it): rows the tile does not own write a spare row,
+  // and the padded steps past either end (a >= -(K - 1), K <= kXsaMaxK)
+  // write slots that hold no pending slice.
+  int xrow[XSA ? R : 1];
+#pragma unroll
+  for (int r = 0; r < (XSA ? R : 0); ++r) xrow[r] = (row_own[r] ? w * R + r - K : XOR) * 64 + lane;
   auto xa_stage = [&](const float (&v)[R], int a) {
     if constexpr (XSA) {
-      if ((unsigned)a < (unsigned)nA) {
-        const int sa = a % kRunSlots;
+      const int sa = (a + kRunSlots) % kRunSlots;
 #pragma unroll
-        for (int r = 0; r < R; ++r)
-          if (row_own[r]) xrun[sa][(w * R + r - K) * 64 + lane] = v[r];
-        // one run completes per block at most (K < kRunLen); a forward
-        // sweep's partial last run is stored after the loop
-        const int g = a & (kRunLen - 1);
-        if (dir < 0 ? g == 0 : g == kRunLen - 1) xa_a0 = a - g;
-      }
+      for (int r = 0; r < R; ++r) xrun[sa][xrow[r]] = v[r];
+      // one run completes per block at most (K < kRunLen); a forward
+      // sweep's partial last run is stored after the loop
+      const int g = a & (kRunLen - 1);
+      const bool done = (unsigned)a < (unsigned)nA && (dir < 0 ? g == 0 : g == kRunLen - 1);
+      xa_a0 = done ? a - g : xa_a0;
     }
   };
 
