@@ -302,10 +302,13 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   // unrolled loops, so edge addresses are immediate offsets)
   constexpr int NWL = NW > 1 ? NW : 1;
   constexpr bool XST = XS == 1, XSA = XS == 2;
-  // XSA needs the LDS for its runs: two edge slots, alternating by step
+  // A-contiguous output of multi-wave tiles (XSW, runs staged in LDS) and of
+  // one-row tiles (XSR, each block's slices held in registers)
+  constexpr bool XSR = XSA && NW == 1 && R == 1, XSW = XSA && !XSR;
+  // XSW needs the LDS for its runs: two edge slots, alternating by step
   // parity (one barrier per step separates a slot's reads from its rewrite
   // two steps later)
-  constexpr int ES = XSA ? 2 : P;
+  constexpr int ES = XSW ? 2 : P;
   __shared__ float edge_first[ES][NWL][64];
   __shared__ float edge_last[ES][NWL][64];
   __shared__ float edgeI_first[ES][NWL][CT][64];
@@ -315,7 +318,8 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   // neighbours are lanes -1, 0, +1 of the front row itself
   constexpr bool ROW1 = NW == 1 && R == 1;
   static_assert(!XST || (NW > 1 && 64 % NWR == 0), "transposed output needs a multi-wave window of 16/32/64 rows");
-  static_assert(!XSA || (NW > 1 && !COMM && !INIT && NWR > 2 * P), "A-contiguous output is a multi-wave, plain-sweep form");
+  static_assert(!XSW || (NW > 1 && !COMM && !INIT && NWR > 2 * P), "A-contiguous output is a plain-sweep form");
+  static_assert(!XSR || (!COMM && !INIT), "one-row deferred stores are a plain-sweep form");
   __shared__ float xtile[XST ? P : 1][XST ? NWR : 1][65];
   // XSA: slot a % kRunSlots holds slice a, one 64-lane row per own row of
   // the tile (window rows K .. K + TB - 1; the host uses this form only on
@@ -323,7 +327,7 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   // kRunPad words apart beyond their rows, so the 16 lanes that read one
   // position's run hit different banks
   constexpr int XOR = NWR - 2 * P;  // own rows at the shallowest halo (+ 1 spare row)
-  __shared__ float xrun[XSA ? kRunSlots : 1][XSA ? (XOR + 1) * 64 + kRunPad : 1];
+  __shared__ float xrun[XSW ? kRunSlots : 1][XSW ? (XOR + 1) * 64 + kRunPad : 1];
   __shared__ unsigned xhist_wg[kXHistBuckets];  // this workgroup's exchange-wait counts (measurement)
   static_assert(!COMM || (NW > 1 && NWR % kCommWaves == 0), "exchange waves serve multi-wave sweeps");
   __shared__ float xfront[COMM ? NWR : 1][64];     // COMM: the front around an exchange
@@ -382,9 +386,10 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   // every K steps; it stays within the sweep only when K is a positive
   // multiple of P.  A launch that breaks this is refused before any store,
   // and so is an A-contiguous output whose own rows or staged slices would
-  // not fit its LDS ring.
+  // not fit its LDS ring, or (one-row tiles) whose block is not one turn of
+  // the slab ring.
   if (K <= 0 || K % P != 0 || (ROW1 && Dd.nB != 1) ||
-      (XSA && (Dd.nTB < 2 || Dd.TB > XOR || K > kRunSlots - kRunLen + 1))) {
+      (XSW && (Dd.nTB < 2 || Dd.TB > XOR || K > kRunSlots - kRunLen + 1)) || (XSR && K != P)) {
     if (threadIdx.x == 0) __hip_atomic_store(status, FG_EINVAL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     return;
   }
@@ -665,7 +670,7 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   // negative window lane are replaced before use).
   int xa_a0 = -1;  // first slice of the run awaiting its stores, or -1
   auto xa_flush = [&]() {
-    if constexpr (XSA) {
+    if constexpr (XSW) {
       constexpr int LPR = kRunLen, PPI = 64 / kRunLen;
       static_assert(64 % kRunLen == 0, "run layout");
       if (xa_a0 >= 0) {
@@ -703,11 +708,11 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   // moving work across it): rows the tile does not own write a spare row,
   // and the padded steps past either end (a >= -(K - 1), K <= kXsaMaxK)
   // write slots that hold no pending slice.
-  int xrow[XSA ? R : 1];
+  int xrow[XSW ? R : 1];
 #pragma unroll
-  for (int r = 0; r < (XSA ? R : 0); ++r) xrow[r] = (row_own[r] ? w * R + r - K : XOR) * 64 + lane;
+  for (int r = 0; r < (XSW ? R : 0); ++r) xrow[r] = (row_own[r] ? w * R + r - K : XOR) * 64 + lane;
   auto xa_stage = [&](const float (&v)[R], int a) {
-    if constexpr (XSA) {
+    if constexpr (XSW) {
       const int sa = (a + kRunSlots) % kRunSlots;
 #pragma unroll
       for (int r = 0; r < R; ++r) xrun[sa][xrow[r]] = v[r];
@@ -716,6 +721,34 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
       const int g = a & (kRunLen - 1);
       const bool done = (unsigned)a < (unsigned)nA && (dir < 0 ? g == 0 : g == kRunLen - 1);
       xa_a0 = done ? a - g : xa_a0;
+    }
+  };
+
+  // XSR (one-row tiles, K == P): the block's new values stay in registers
+  // (slot p = block step p) and are stored at the next exchange, after the
+  // publications, element (a, e) at Dd.dxo[e * sXe + a * sXa]: the other 2D
+  // layout (sXe = nA, sXa = 1), each own lane writing a run of K consecutive
+  // elements.  (Deferring the stores of sweeps into their own array this way
+  // measured no faster.)  Slices past either end of the
+  // sweep (the padded last block) and lanes the tile does not own get an
+  // out-of-range offset.  The host uses this form only for arrays of less
+  // than 2 GiB (nA * nE elements).
+  float xrr[XSR ? P : 1];
+  int xr_s0 = 0;  // first step of the block held in xrr
+  const unsigned xr_range = XSR ? (unsigned)((long long)nA * nE * 4) : 0u;
+  const unsigned xr_lane = XSR ? (unsigned)e * (unsigned)Dd.sXe * 4u : 0u;  // own lanes: e >= 0
+  const unsigned xr_sa4 = XSR ? (unsigned)Dd.sXa * 4u : 0u;
+  auto xr_store = [&](float v, int a) {
+    if constexpr (XSR) {
+      const __amdgpu_buffer_rsrc_t rx = __builtin_amdgcn_make_buffer_rsrc((void *)Dd.dxo, 0, xr_range, 0x00020000);
+      const unsigned off = (lane_own && (unsigned)a < (unsigned)nA) ? xr_lane + (unsigned)a * xr_sa4 : 0xFFFFFFFFu;
+      __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), rx, (int)off, 0, 0);
+    }
+  };
+  auto xr_flush = [&]() {
+    if constexpr (XSR) {
+#pragma unroll
+      for (int p = 0; p < P; ++p) xr_store(xrr[p], a_first + (xr_s0 + p) * dir);
     }
   };
 
@@ -743,6 +776,7 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     for (int r = 0; r < R; ++r) xtile[P - 1][w * R + r][lane] = Fd[r];
   }
   xa_stage(Fd, a_first - dir);  // XSA: the sweep's untouched first slice
+  xr_store(Fd[0], a_first - dir);
 #pragma unroll
   for (int p = 0; p < P - 1; ++p) load_slab(S[p], cursor_of(p), p);
   // the slices that the loop stores (step s) and prefetches (step s + P - 1)
@@ -797,11 +831,12 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
         __hip_atomic_store(pub + (long long)(b - b_own0) * own_w + lane_pub, pack(Fd[r], tag), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     }
-    // XSA: a run completed in this block is stored after the publications
-    // (which then leave first) and drains while the wave waits to poll; the
-    // halo wait below also waits for it, so every path leaves the exchange
-    // with the same loads in flight
+    // XSA: a run completed in this block (one-row tiles: the block) is
+    // stored after the publications (which then leave first) and drains
+    // while the wave waits to poll; the halo wait below also waits for it,
+    // so every path leaves the exchange with the same loads in flight
     xa_flush();
+    xr_flush();
     // Neighbours publish at about the same moment; polling right away mostly
     // finds their values not yet visible and costs a second round trip.
     // Waiting ~1500 clocks first was measured fastest for multi-wave tiles
@@ -871,7 +906,7 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     // the distance; their outer image rows then only meet +inf candidates)
     float top = kInf, bot = kInf;
     if constexpr (NW > 1) {
-      const int es = XSA ? (s & 1) : slot;
+      const int es = XSW ? (s & 1) : slot;
       edge_first[es][w][lane] = Fd[0];
       edge_last[es][w][lane] = Fd[R - 1];
 #pragma unroll
@@ -925,8 +960,10 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     if constexpr (XST) {
 #pragma unroll
       for (int r = 0; r < R; ++r) xtile[slot][w * R + r][lane] = nd[r];
-    } else if constexpr (XSA) {
+    } else if constexpr (XSW) {
       xa_stage(nd, cur_st.a);
+    } else if constexpr (XSR) {
+      xrr[slot] = nd[0];  // K == P: slot is the step's index in its block
     } else {
       // a padded step's slice is outside the array: all its stores are dropped
       const __amdgpu_buffer_rsrc_t rs = cursor_rsrc(Dd.d, cur_st);
@@ -957,6 +994,7 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
   const int nblocks = (nsteps + K - 1) / K;
   for (int blk = 0;; ++blk) {
     const int s0 = blk * K;
+    xr_s0 = s0;
     for (int g = 0; g < groups - 1; ++g) {
 #pragma unroll
       for (int p = 0; p < P; ++p) step(S[p], S[(p + P - 1) % P], s0 + g * P + p, p, true);
@@ -974,7 +1012,8 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     __syncthreads();
     xflush(P - 1, nblocks * K - 1);  // the last step sits in slot P - 1
   }
-  if constexpr (XSA) {
+  if constexpr (XSR) xr_flush();  // the last block
+  if constexpr (XSW) {
     xa_flush();  // a run completed in the last block
     if (dir > 0 && ((nA - 1) & (kRunLen - 1)) != kRunLen - 1) {  // a forward sweep's partial last run
       xa_a0 = (nA - 1) & ~(kRunLen - 1);
@@ -1506,17 +1545,32 @@ const void *pick_kernel_nr(int C, bool lamb1, bool fast, bool init) {
   return nullptr;
 }
 
+// One-row kernels with deferred stores into the other 2D layout on the ring
+// P, 1-4 channels.
+template <int P>
+const void *pick_row_deferred(int C, bool lamb1, bool fast) {
+  switch (C) {
+    case 1: return pick_variant<1, 1, P, 1, 2>(lamb1, fast);
+    case 2: return pick_variant<1, 1, P, 2, 2>(lamb1, fast);
+    case 3: return pick_variant<1, 1, P, 3, 2>(lamb1, fast);
+    case 4: return pick_variant<1, 1, P, 4, 2>(lamb1, fast);
+    default: return nullptr;
+  }
+}
+
 // The kernel compiled for exactly this (NW, R, P) and channel count, or null.
 // xs selects the output form (0 own layout, 1 B contiguous, 2 A contiguous).
 // B-contiguous kernels exist for the two 5-slice-ring 3D shapes: 8 x 4 with
 // 1-2 channels, 16 x 2 with 1 channel (2 channels would exceed the LDS);
-// A-contiguous ones for both shapes with 1 channel.
+// A-contiguous ones for both shapes with 1 channel, and for one-row tiles
+// (deferred stores) on the 30-slice ring with 1-4 channels.
 constexpr int kXstMaxC = 2;
 constexpr int kXsaMaxC = 1;
 constexpr int kXsaMaxK = kRunSlots - kRunLen + 1;
 const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast, int xs = 0, bool init = false,
                         bool comm = false) {
   if (xs && init) return nullptr;
+  if (xs == 2 && NW == 1 && R == 1) return (P == kRowDeepK && !comm) ? pick_row_deferred<kRowDeepK>(C, lamb1, fast) : nullptr;
   if (comm) return (NW == 8 && R == 4 && P == 5) ? pick_comm_kernel(C, lamb1, fast, xs, init) : nullptr;
   if (xs == 1) {
     if (NW == 8 && R == 4 && P == 5 && C == 1) return pick_variant<8, 4, 5, 1, 1>(lamb1, fast);
@@ -1858,6 +1912,11 @@ struct Runner {
       s.dxo = x.dT;
       s.sXe = (long long)s.nA * s.nB;
       s.sXa = s.nB;
+    } else if (xs == 2 && x.is2d) {  // rows (A = H, E = W) written as (W, H), columns as (H, W)
+      s.dxo = pass == 1 ? x.dT : x.d;
+      s.sXe = s.nA;
+      s.sXa = 1;
+      s.sXb = 0;
     } else if (xs == 2) {  // W pass (A = W, B = H, E = D) written as (D, H, W)
       s.dxo = x.d;
       s.sXe = (long long)x.H * x.W;
@@ -1870,7 +1929,8 @@ struct Runner {
   // transposed-output kernel for its channel count, and every output offset
   // within 32 bits.
   bool xstore_candidate(const Xform &x) {
-    if (x.is2d || engine == 0 || x.C > kXstMaxC) return false;
+    if (x.is2d) return row_xsa_candidate(x, 1);
+    if (engine == 0 || x.C > kXstMaxC) return false;
     int nA, nB, nE;
     pass_plane(x, 1, nA, nB, nE);
     if (nA <= 1 || nB <= 1) return false;
@@ -1887,13 +1947,27 @@ struct Runner {
   // the last iteration of a transform whose result leaves through a
   // combining transpose (signed and GSF epilogues).
   bool xsa_candidate(const Xform &x) {
-    if (x.is2d || engine == 0 || x.C > kXsaMaxC) return false;
     if (x.it == x.iters - 1 && (x.fin_member || x.fin_kind != XK_COPY)) return false;
+    if (x.is2d) return row_xsa_candidate(x, 2);
+    if (engine == 0 || x.C > kXsaMaxC) return false;
     int nA, nB, nE;
     pass_plane(x, 2, nA, nB, nE);
     if (nA <= 1 || nB <= 1) return false;
     if ((long long)x.D * x.H * x.W * 4 >= 0xFFFFFFF0ll) return false;
     return slice_offsets_fit(x, 2);
+  }
+
+  // 2D: whether the backward sweep of rows (pass 1) may write the column
+  // layout, or that of columns (pass 2) the native layout, itself; the
+  // launch then also needs the one-row kernel whose block is one turn of its
+  // 30-slice ring (checked when it is recorded).  The output, like the input,
+  // is addressed with 32-bit offsets within 2^31 bytes.
+  bool row_xsa_candidate(const Xform &x, int pass) {
+    if (engine == 0 || x.C > 4) return false;
+    int nA, nB, nE;
+    pass_plane(x, pass, nA, nB, nE);
+    if (nA <= 1) return false;
+    return slice_offsets_fit(x, 1) && slice_offsets_fit(x, 2);
   }
 
   // The tiled kernel addresses elements of a slice with 32-bit byte offsets
@@ -2122,18 +2196,35 @@ struct Runner {
         if (ks[i] == 0) {
           if (in) return fail(FG_EINVAL, "internal: an initialising sweep was planned on the per-slice engine");
           record_per_slice(*x, pass, dir);
+          if (pass == 1) x->xst = false;  // its output stays in the native layout
           if (pass == 2) x->xsa = false;  // its output stays in the W pass layout
           continue;
         }
         const Plan p = plan_fixed(nB, nE, ks[i], x->C);
-        int xs = (pass == 1 && dir < 0 && x->xst) ? 1 : 0;
+        // A-contiguous output: 3D tiles stage the own rows of a tile of
+        // several along B, and the slices of one block beyond a run; one-row
+        // tiles hold one ring turn per block.  Otherwise the distance moves
+        // between layouts through a transpose.
+        // Backward sweeps that write the next layout themselves: 3D tiles
+        // stage the own rows of a tile of several along B, and the slices of
+        // one block beyond a run; 2D one-row tiles defer a block's stores to
+        // the exchange, on the 30-slice ring with one ring turn per block.
+        // Otherwise the distance moves between layouts through a transpose.
+        const bool row_other = x->is2d && p.NW == 1 && p.R == 1 && p.P == kRowDeepK && p.K == p.P &&
+                               pick_kernel(p.NW, p.R, p.P, x->C, x->lamb1, x->fast, 2, in, p.comm);
+        int xs = 0;
+        if (pass == 1 && dir < 0 && x->xst) {
+          if (!x->is2d) xs = 1;
+          else if (row_other) xs = 2;
+          else x->xst = false;
+        }
         if (pass == 2 && dir < 0 && x->xsa) {
-          // the A-contiguous form stages the own rows of a tile of several
-          // along B, and the slices of one block beyond a run; otherwise the
-          // distance leaves through a transpose
-          if (p.nTB > 1 && p.K <= kXsaMaxK && pick_kernel(p.NW, p.R, p.P, x->C, x->lamb1, x->fast, 2, in, p.comm))
+          if (x->is2d ? row_other
+                      : (p.nTB > 1 && p.K <= kXsaMaxK &&
+                         pick_kernel(p.NW, p.R, p.P, x->C, x->lamb1, x->fast, 2, in, p.comm)))
             xs = 2;
-          else x->xsa = false;
+          else
+            x->xsa = false;
         }
         bool found = false;
         for (auto &g : groups)

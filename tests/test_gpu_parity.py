@@ -147,7 +147,9 @@ def test_random_3d_vs_oracle(shape, spacing, C, lamb, it, engine):
 
 
 RANDOM_2D = [((300, 257), 1, 1.0, 2), ((61, 62), 3, 0.5, 2), ((1, 700), 1, 0.7, 3),
-             ((513, 1), 2, 0.2, 2), ((128, 1000), 3, 0.0, 1), ((97, 130), 4, 1.0, 4)]
+             ((513, 1), 2, 0.2, 2), ((128, 1000), 3, 0.0, 1), ((97, 130), 4, 1.0, 4),
+             # row and column sweeps of exactly one 30-slice block, and of two
+             ((31, 200), 1, 1.0, 2), ((61, 31), 2, 0.5, 3)]
 
 
 @pytest.mark.parametrize("shape,C,lamb,it", RANDOM_2D)
