@@ -14,9 +14,13 @@
 //   H pass: A=H, B=D, E=W on the native array
 //   W pass: A=W, B=H, E=D on a transposed (W,H,D) copy (image transposed once
 //           per call; the distance enters that layout through the H pass's
-//           backward sweep, which writes it directly (or, for 3-4 channels and
-//           non-default shapes, through a transpose) and leaves it through a
-//           transpose after the W pass pair)
+//           backward sweep, which writes it directly, and leaves it through
+//           the W pass's backward sweep, which writes the native layout; a
+//           transpose instead where those forms do not apply (3-4 channels,
+//           other shapes and halo depths) and after a signed or GSF
+//           transform's last W pass, where it applies the epilogue)
+//   2D: rows A=H, E=W on (H,W); columns A=W, E=H on (W,H), the backward row
+//       and column sweeps on the 30-slice ring writing the other layout
 // B and E are the reference's in-plane axes in its neighbour order (B outer,
 // E inner), so the exact-order path visits candidates in the reference order.
 //

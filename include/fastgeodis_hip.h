@@ -181,8 +181,8 @@ int fg_geodesic_batch_f32(const fg_instance *inst, int n, void *workspace,
  * transforms (iterations >= 1, first sweep on the tiled engine) are added at
  * any time; each fg_stepper_step records, for every active transform, its
  * next pass pair (forward and backward sweep) in launches shared with every
- * other active transform at that pass, plus the transposes around W passes,
- * and enqueues them on `stream`.  A transform whose last W pass ran in a
+ * other active transform at that pass, plus the transposes around W passes
+ * that its sweeps do not replace, and enqueues them on `stream`.  A transform whose last W pass ran in a
  * step is reported in `done` (its ticket) and has left the stepper; its
  * output is complete when `stream` reaches the end of that step.  Results
  * equal the same instance run alone.
