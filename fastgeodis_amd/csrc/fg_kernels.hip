@@ -2205,10 +2205,6 @@ struct Runner {
           continue;
         }
         const Plan p = plan_fixed(nB, nE, ks[i], x->C);
-        // A-contiguous output: 3D tiles stage the own rows of a tile of
-        // several along B, and the slices of one block beyond a run; one-row
-        // tiles hold one ring turn per block.  Otherwise the distance moves
-        // between layouts through a transpose.
         // Backward sweeps that write the next layout themselves: 3D tiles
         // stage the own rows of a tile of several along B, and the slices of
         // one block beyond a run; 2D one-row tiles defer a block's stores to
