@@ -285,9 +285,12 @@ struct Slab {  // one prefetched slice: image rows -1..R, distance rows 0..R-1
 //   1 (XST): writes it to Dd.dxo with B contiguous (the H pass writes the
 //     (W, H, D) layout of the W pass), staging each slice's window through
 //     LDS so each store covers a run of consecutive B.
-//   2 (XSA): writes it to Dd.dxo with A contiguous (the W pass writes the
-//     native (D, H, W) layout), staging kRunLen consecutive slices per wave
-//     in LDS so that each store covers runs of kRunLen consecutive A.
+//   2 (XSA): writes it to Dd.dxo with A contiguous: multi-wave tiles (the W
+//     pass writing the native (D, H, W) layout) stage kRunLen consecutive
+//     slices per wave in LDS so that each store covers runs of kRunLen
+//     consecutive A; one-row tiles on the deep ring (2D rows writing the
+//     column layout and back) hold each block in registers and store a run
+//     of K elements per own lane at the exchange.
 // INIT: the first sweep of a transform, whose input distance is computed from
 // the mask as it is loaded (SweepDesc::init_mode).
 // COMM: kCommWaves more waves (indices NW, NW + 1, ...) that hold no rows and
