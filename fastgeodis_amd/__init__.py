@@ -229,8 +229,8 @@ def set_tuning(engine: int = 1, halo: int = 0, max_groups: int = 0, shape: int =
     per compute unit), workgroup cap (0 = occupancy limit) and 3D plane
     workgroup shape (0 auto = 5, 1 = 4 waves x 8 rows, 2 = 8 x 4, 3 = 16 x 2,
     4 = 8 x 4 with a deeper prefetch ring, 5 = 16 x 2 with the deeper ring,
-    6 = 8 x 4 with the deeper ring and one more wave that performs the halo
-    exchanges)
+    6 = 8 x 4 with the deeper ring and four more waves that perform the halo
+    exchanges, 7 = 8 waves x 2 rows, a 16-row window, with the deeper ring)
     used by later calls on this host thread."""
     FastGeodisCpp.set_tuning(int(engine), int(halo), int(max_groups), int(shape))
 

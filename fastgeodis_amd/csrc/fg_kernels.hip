@@ -1428,20 +1428,31 @@ struct Shape {
 // (one row); the others tile 3D planes (selected by fg_tuning.shape).
 constexpr Shape kShapes[] = {{1, 1, 6, 6, false},  {4, 8, 2, 2, false},  {8, 4, 3, 2, false},
                              {16, 2, 3, 3, false}, {8, 4, 5, 2, false},  {16, 2, 5, 3, false},
-                             {8, 4, 5, 2, true}};
-constexpr int kNumShapes = 7;
+                             {8, 4, 5, 2, true},   {8, 2, 5, 3, false}};
+constexpr int kNumShapes = 8;
 // One-row planes at K = 30 (a lone row, or rows few enough for one tile per
 // CU at that depth): the whole block in the slab ring, so the prefetches of
 // a block are never waited for at a loop head.
 constexpr Shape kRowDeep = {1, 1, 30, 30, false};
 constexpr int kRowDeepK = 30;
 constexpr int kAutoShape = 5;  // fastest on 512^2 planes: 16 x 2 with the deepest ring that keeps K = 5
+// 16-row windows (8 x 2): with the automatic shape, a call of one unsigned
+// volume (1-2 channels, general candidate: lamb != 1) whose planes fill
+// between half the compute units and all of them with these tiles at K = 5
+// runs on them: the 32-row windows leave most compute units idle on such
+// planes (128 x 256 x 256: 66-132 tiles) and compute 3.9 x their own tile
+// at K = 10.  Measured on that volume: 5.49 -> 4.68 ms (lamb 0.5), 5.52 ->
+// 4.72 ms (lamb 0); lamb 1 unchanged (3.97 / 3.98 ms, bound by the loads
+// rather than the arithmetic); signed calls (two planes per pass) and
+// batches ran slower with them (4.18 -> 4.55 ms, 20.7 -> 27.2 ms).
+constexpr int kSmallPlaneShape = 7;
 thread_local int g_shape = kAutoShape;
 
-Plan plan_fixed(int nB, int nE, int Kreq, int C) {
+Plan plan_fixed(int nB, int nE, int Kreq, int C, int shape = -1) {
   Plan p{};
   p.ok = false;
-  const Shape s = (nB == 1) ? (Kreq > kRowDeepK - kShapes[0].P12 ? kRowDeep : kShapes[0]) : kShapes[g_shape];
+  const Shape s = (nB == 1) ? (Kreq > kRowDeepK - kShapes[0].P12 ? kRowDeep : kShapes[0])
+                            : kShapes[shape >= 0 ? shape : g_shape];
   p.NW = s.NW;
   p.R = s.R;
   p.comm = s.comm && C <= 2;  // exchange-wave kernels exist for 1-2 channels; 3-4 run the plain 8 x 4
@@ -1600,6 +1611,8 @@ const void *pick_kernel(int NW, int R, int P, int C, bool lamb1, bool fast, int 
   FG_S(16, 2, 5, 1, 2)
   FG_S(8, 4, 2, 3, 4)
   FG_S(16, 2, 3, 1, 4)
+  FG_S(8, 2, 5, 1, 2)
+  FG_S(8, 2, 3, 3, 4)
 #undef FG_S
   return nullptr;
 }
@@ -2614,9 +2627,11 @@ size_t halo_entries_for(const std::vector<OpSpec> &ops, int K) {
       int nA, nB, nE;
       pass_plane(x, p, nA, nB, nE);
       for (int k = (K > 0 ? K : 1); k <= (K > 0 ? K : kMaxK); ++k) {
-        Plan pl = plan_fixed(nB, nE, k, o.C);
-        if (!pl.ok) continue;
-        worst = std::max(worst, (size_t)pl.nTB * pl.nTE * 2 * pl.TB * pl.TE);
+        for (int sh : {g_shape, g_shape == kAutoShape ? kSmallPlaneShape : g_shape}) {
+          Plan pl = plan_fixed(nB, nE, k, o.C, sh);
+          if (!pl.ok) continue;
+          worst = std::max(worst, (size_t)pl.nTB * pl.nTE * 2 * pl.TB * pl.TE);
+        }
       }
     }
     total += worst * (size_t)n_dist(o.op);
@@ -2689,6 +2704,34 @@ struct RunOpts {
   int *fallback_status = nullptr;  // status word to use while the device has no pinned one
 };
 
+// Whether a call runs its 3D planes on the 16-row windows (kSmallPlaneShape).
+bool small_volume_call(const std::vector<OpSpec> &ops, int cus) {
+  if (g_shape != kAutoShape || ops.size() != 1) return false;
+  const OpSpec &o = ops[0];
+  if (o.is2d || o.op != FG_OP_UNSIGNED || o.C > 2 || o.iters < 1) return false;
+  Xform x{};
+  make_xform(o, o.lg, o.le, x);
+  if (x.lamb1) return false;
+  long long most = 0;
+  for (int pass = 0; pass < 3; ++pass) {
+    int nA, nB, nE;
+    pass_plane(x, pass, nA, nB, nE);
+    if (nA <= 1 || nB <= 1) continue;
+    const Plan p = plan_fixed(nB, nE, 5, o.C, kSmallPlaneShape);
+    const long long tiles = (long long)p.nTB * p.nTE;
+    if (!p.ok || tiles > cus) return false;
+    most = std::max(most, tiles);
+  }
+  return 2 * most > cus;
+}
+
+// Sets this thread's shape for one call and restores it on every way out.
+struct ShapeScope {
+  int saved;
+  explicit ShapeScope(int shape) : saved(g_shape) { g_shape = shape; }
+  ~ShapeScope() { g_shape = saved; }
+};
+
 int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream, const RunOpts &opt = RunOpts()) {
   for (auto &o : ops) {
     int rc = validate(o);
@@ -2705,6 +2748,7 @@ int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream, 
   Runner run{};
   int rc = query_device(run.dev);
   if (rc) return rc;
+  ShapeScope shape_scope(small_volume_call(ops, run.dev.cus) ? kSmallPlaneShape : g_shape);
   // an expiry left by an earlier call on this device is reported here
   DeviceState &dstate = device_state(run.dev.id);
   if (!opt.no_alloc && (rc = ensure_status_word(dstate))) return rc;
@@ -3335,7 +3379,7 @@ int fg_tuning_set(const fg_tuning *t) {
   if (t->engine < 0 || t->engine > 1) return fail(FG_EINVAL, "engine must be 0 or 1");
   if (t->halo < 0 || t->halo > kMaxK) return fail(FG_EINVAL, "halo must be 0 (automatic) or in [1, 30]");
   if (t->max_groups < 0) return fail(FG_EINVAL, "max_groups must be >= 0");
-  if (t->shape < 0 || t->shape >= kNumShapes) return fail(FG_EINVAL, "shape must be in [0, 6]");
+  if (t->shape < 0 || t->shape >= kNumShapes) return fail(FG_EINVAL, "shape must be in [0, 7]");
   g_tune = *t;
   g_shape = t->shape == 0 ? kAutoShape : t->shape;
   return FG_OK;

@@ -232,7 +232,8 @@ typedef struct fg_tuning {
   int shape;      /* 3D plane workgroup shape (waves x rows / slab ring): 0 auto
                      (= 5), 1 = 4x8/2, 2 = 8x4/3, 3 = 16x2/3, 4 = 8x4/5, 5 = 16x2/5,
                      6 = 8x4/5 plus four waves that perform the halo exchanges
-                     (1-2 channels; 3-4 channels run 8x4/2) */
+                     (1-2 channels; 3-4 channels run 8x4/2), 7 = 8x2/5 (16-row
+                     windows; 3-4 channels 8x2/3) */
 } fg_tuning;
 int fg_tuning_set(const fg_tuning *t);
 int fg_tuning_get(fg_tuning *t);

@@ -117,8 +117,10 @@ def test_tuning_roundtrip():
     try:
         FG.set_tuning(0, 3, 17, 3)
         assert FG.get_tuning() == {"engine": 0, "halo": 3, "max_groups": 17, "shape": 3}
+        FG.set_tuning(1, 4, 0, 7)  # the last shape
+        assert FG.get_tuning()["shape"] == 7
         with pytest.raises(ValueError):
-            FG.set_tuning(1, 4, 0, 7)
+            FG.set_tuning(1, 4, 0, 8)
         with pytest.raises(ValueError):
             FG.set_tuning(2, 4, 0)
         FG.set_tuning(1, 0, 0)  # automatic halo depth

@@ -46,16 +46,18 @@ def diff_report(a, b):
 
 # (engine, halo K, shape); K is rounded up to a multiple of the shape's slab
 # ring (4x8: 2, 8x4: 3, 16x2: 3, 8x4 deep: 5, 16x2 deep: 5, 8x4 deep with
-# exchange waves: 5; shape 0 = 16x2 deep; one-row planes: 6, or 30 for
-# K > 24), K = 0 picks it per plane
+# exchange waves: 5, 8x2 deep: 5; shape 0 = 16x2 deep; one-row planes: 6, or
+# 30 for K > 24), K = 0 picks it per plane
 ENGINES = [(1, 0, 0), (1, 4, 0), (1, 1, 1), (1, 3, 3), (1, 7, 2), (1, 2, 1), (1, 5, 3), (1, 5, 4),
-           (1, 10, 4), (1, 0, 4), (1, 10, 5), (0, 4, 0), (1, 0, 6), (1, 10, 6), (1, 24, 0), (1, 30, 0)]
+           (1, 10, 4), (1, 0, 4), (1, 10, 5), (0, 4, 0), (1, 0, 6), (1, 10, 6), (1, 24, 0), (1, 30, 0),
+           (1, 0, 7), (1, 5, 7)]
 
 
 @pytest.fixture(params=ENGINES, ids=["tiled-auto", "tiled-K4-auto", "tiled-K1-4x8", "tiled-K3-16x2",
                                      "tiled-K7-8x4", "tiled-K2-4x8", "tiled-K5-16x2", "tiled-K5-8x4deep",
                                      "tiled-K10-8x4deep", "tiled-auto-8x4deep", "tiled-K10-16x2deep", "per-slice",
-                                     "tiled-auto-8x4xchg", "tiled-K10-8x4xchg", "tiled-K24", "tiled-K30"])
+                                     "tiled-auto-8x4xchg", "tiled-K10-8x4xchg", "tiled-K24", "tiled-K30",
+                                     "tiled-auto-8x2", "tiled-K5-8x2"])
 def engine(request, device):
     e, k, shape = request.param
     old = FG.get_tuning()
