@@ -37,7 +37,7 @@ from . import generalised_geodesic2d, generalised_geodesic3d, get_tuning, set_tu
 # (engine, halo, shape): halo 0 = automatic depth, shape 0 = default
 DEFAULT_CANDIDATES_3D: Tuple[Tuple[int, int, int], ...] = (
     (1, 0, 5), (1, 0, 4), (1, 0, 2), (1, 0, 3), (1, 0, 1),
-    (1, 3, 2), (1, 6, 2), (1, 10, 4), (1, 8, 1),
+    (1, 3, 2), (1, 6, 2), (1, 10, 4), (1, 8, 1), (1, 0, 7),
 )
 DEFAULT_CANDIDATES_2D: Tuple[Tuple[int, int, int], ...] = (
     (1, 0, 0), (1, 6, 0), (1, 12, 0), (1, 24, 0), (1, 30, 0),
