@@ -1436,17 +1436,27 @@ constexpr int kNumShapes = 8;
 constexpr Shape kRowDeep = {1, 1, 30, 30, false};
 constexpr int kRowDeepK = 30;
 constexpr int kAutoShape = 5;  // fastest on 512^2 planes: 16 x 2 with the deepest ring that keeps K = 5
-// 16-row windows (8 x 2): with the automatic shape, a call of one unsigned
-// volume (1-2 channels, general candidate: lamb != 1) whose planes fill
-// between half the compute units and all of them with these tiles at K = 5
-// runs on them: the 32-row windows leave most compute units idle on such
-// planes (128 x 256 x 256: 66-132 tiles) and compute 3.9 x their own tile
-// at K = 10.  Measured on that volume: 5.49 -> 4.68 ms (lamb 0.5), 5.52 ->
-// 4.72 ms (lamb 0); lamb 1 unchanged (3.97 / 3.98 ms, bound by the loads
-// rather than the arithmetic); signed calls (two planes per pass) and
-// batches ran slower with them (4.18 -> 4.55 ms, 20.7 -> 27.2 ms).
+// 16-row windows (8 x 2): with the automatic shape and halo, a pass whose planes (3D,
+// 1-2 channels, general candidate: lamb != 1) all fit the compute units with
+// these tiles at K = 5, and then occupy at least 1.5 x as many of them as
+// with the 32-row windows, runs on them (see Runner::pass_shape): the
+// 32-row windows leave most compute units idle on mid-size planes
+// (128 x 256 x 256: 66-132 tiles) and compute 3.9 x their own tile at
+// K = 10.  Measured on that volume (every pass on 16-row windows): 5.49 ->
+// 4.68 ms (lamb 0.5), 5.52 -> 4.72 ms (lamb 0); lamb 1 unchanged
+// (3.97 / 3.98 ms, bound by the loads rather than the arithmetic); a
+// signed call with every pass on them, including the D pass whose two
+// planes then exceed the device, and batches ran slower (4.18 -> 4.55 ms,
+// 20.7 -> 27.2 ms).
 constexpr int kSmallPlaneShape = 7;
 thread_local int g_shape = kAutoShape;
+
+// Sets this thread's shape for one pass and restores it on every way out.
+struct ShapeScope {
+  int saved;
+  explicit ShapeScope(int shape) : saved(g_shape) { g_shape = shape; }
+  ~ShapeScope() { g_shape = saved; }
+};
 
 Plan plan_fixed(int nB, int nE, int Kreq, int C, int shape = -1) {
   Plan p{};
@@ -2201,8 +2211,30 @@ struct Runner {
     }
   }
 
+  // The 3D workgroup shape of a pass: the tuned one, or with the automatic
+  // shape the 16-row windows when the pass's planes qualify (kSmallPlaneShape).
+  int pass_shape(const std::vector<Xform *> &active, int pass) {
+    if (g_tune.shape != 0 || K != 0 || engine == 0) return g_shape;  // tuned shape or halo
+    const std::vector<int> ks = pass_halos(active, pass);  // with the 32-row windows
+    long long t32 = 0, t16 = 0;
+    for (size_t i = 0; i < active.size(); ++i) {
+      const Xform *x = active[i];
+      int nA, nB, nE;
+      pass_plane(*x, pass, nA, nB, nE);
+      if (nA <= 1) continue;
+      if (x->is2d || nB <= 1 || x->C > 2 || x->lamb1 || ks[i] == 0) return g_shape;
+      const Plan a = plan_fixed(nB, nE, ks[i], x->C);
+      const Plan b = plan_fixed(nB, nE, 5, x->C, kSmallPlaneShape);
+      if (!a.ok || !b.ok) return g_shape;
+      t32 += (long long)a.nTB * a.nTE;
+      t16 += (long long)b.nTB * b.nTE;
+    }
+    return (t16 > 0 && t16 <= dev.cus && 2 * t16 >= 3 * t32) ? kSmallPlaneShape : g_shape;
+  }
+
   // Record one pass (forward then backward) for the active transforms.
   int sweep_pass(std::vector<Xform *> &active, int pass) {
+    ShapeScope shape_scope(pass_shape(active, pass));
     const std::vector<int> ks = pass_halos(active, pass);
     for (int dir : {+1, -1}) {
       struct G { int NW, R, P, K, C; bool l1, fa; int xs; bool in, cm; std::vector<SweepJob> jobs; };
@@ -2230,8 +2262,9 @@ struct Runner {
                                pick_kernel(p.NW, p.R, p.P, x->C, x->lamb1, x->fast, 2, in, p.comm);
         int xs = 0;
         if (pass == 1 && dir < 0 && x->xst) {
-          if (!x->is2d) xs = 1;
-          else if (row_other) xs = 2;
+          // (the pass may run on a shape without a B-contiguous kernel)
+          if (!x->is2d && pick_kernel(p.NW, p.R, p.P, x->C, x->lamb1, x->fast, 1, in, p.comm)) xs = 1;
+          else if (x->is2d && row_other) xs = 2;
           else x->xst = false;
         }
         if (pass == 2 && dir < 0 && x->xsa) {
@@ -2627,7 +2660,7 @@ size_t halo_entries_for(const std::vector<OpSpec> &ops, int K) {
       int nA, nB, nE;
       pass_plane(x, p, nA, nB, nE);
       for (int k = (K > 0 ? K : 1); k <= (K > 0 ? K : kMaxK); ++k) {
-        for (int sh : {g_shape, g_shape == kAutoShape ? kSmallPlaneShape : g_shape}) {
+        for (int sh : {g_shape, (g_tune.shape == 0 && K <= 0) ? kSmallPlaneShape : g_shape}) {
           Plan pl = plan_fixed(nB, nE, k, o.C, sh);
           if (!pl.ok) continue;
           worst = std::max(worst, (size_t)pl.nTB * pl.nTE * 2 * pl.TB * pl.TE);
@@ -2704,34 +2737,6 @@ struct RunOpts {
   int *fallback_status = nullptr;  // status word to use while the device has no pinned one
 };
 
-// Whether a call runs its 3D planes on the 16-row windows (kSmallPlaneShape).
-bool small_volume_call(const std::vector<OpSpec> &ops, int cus) {
-  if (g_shape != kAutoShape || ops.size() != 1) return false;
-  const OpSpec &o = ops[0];
-  if (o.is2d || o.op != FG_OP_UNSIGNED || o.C > 2 || o.iters < 1) return false;
-  Xform x{};
-  make_xform(o, o.lg, o.le, x);
-  if (x.lamb1) return false;
-  long long most = 0;
-  for (int pass = 0; pass < 3; ++pass) {
-    int nA, nB, nE;
-    pass_plane(x, pass, nA, nB, nE);
-    if (nA <= 1 || nB <= 1) continue;
-    const Plan p = plan_fixed(nB, nE, 5, o.C, kSmallPlaneShape);
-    const long long tiles = (long long)p.nTB * p.nTE;
-    if (!p.ok || tiles > cus) return false;
-    most = std::max(most, tiles);
-  }
-  return 2 * most > cus;
-}
-
-// Sets this thread's shape for one call and restores it on every way out.
-struct ShapeScope {
-  int saved;
-  explicit ShapeScope(int shape) : saved(g_shape) { g_shape = shape; }
-  ~ShapeScope() { g_shape = saved; }
-};
-
 int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream, const RunOpts &opt = RunOpts()) {
   for (auto &o : ops) {
     int rc = validate(o);
@@ -2748,7 +2753,6 @@ int run_ops(std::vector<OpSpec> &ops, void *ws, size_t wsb, hipStream_t stream, 
   Runner run{};
   int rc = query_device(run.dev);
   if (rc) return rc;
-  ShapeScope shape_scope(small_volume_call(ops, run.dev.cus) ? kSmallPlaneShape : g_shape);
   // an expiry left by an earlier call on this device is reported here
   DeviceState &dstate = device_state(run.dev.id);
   if (!opt.no_alloc && (rc = ensure_status_word(dstate))) return rc;
