@@ -72,9 +72,9 @@ constexpr float kInf = __builtin_huge_valf();
 constexpr int kMaxK = 30;
 // s_sleep units (64 clocks each) between publishing a halo and first polling
 // the neighbours' publications.
-constexpr int kHaloPollDelay = 24;
+constexpr int kHaloPollDelay = 20;
 // The same for 16-row windows (measured: 12 on 128 x 256 x 256 volumes, 3 %
-// faster than 24; 24 stays best for the 32-row windows of 512^2 planes).
+// faster than 24; 12 made the 32-row windows of 512^2 planes 1 % slower).
 constexpr int kHaloPollDelay16 = 12;
 // The same for one-wave row tiles (two neighbours each): 512 clocks.
 constexpr int kRowPollDelay = 8;
@@ -849,8 +849,9 @@ __global__ __launch_bounds__((NW + (COMM ? kCommWaves : 0)) * 64) void sweep_til
     xr_flush();
     // Neighbours publish at about the same moment; polling right away mostly
     // finds their values not yet visible and costs a second round trip.
-    // Waiting ~1500 clocks first was measured fastest for multi-wave tiles
-    // (16..24 x 64 clocks), 512 clocks for one-wave row tiles.
+    // Waiting ~1300 clocks first was measured fastest for 32-row tiles
+    // (16..28 x 64 clocks: 20 best by ~0.2 %, 28 0.5 % slower), ~770 for
+    // 16-row tiles and 512 for one-wave row tiles.
     if constexpr (NW == 1) __builtin_amdgcn_s_sleep(kRowPollDelay);
     else if constexpr (NWR <= 16) __builtin_amdgcn_s_sleep(kHaloPollDelay16);
     else __builtin_amdgcn_s_sleep(kHaloPollDelay);
